@@ -1,0 +1,52 @@
+/* Device-resident and batched entry points of the MI355X ERI engine.
+ *
+ * These have no counterpart in the reference's API; they sit next to the
+ * drop-in simint_compute_eri for callers that keep shell-pair data and
+ * integrals in GPU memory (HBM) and submit many quartets per call.  Device
+ * pointers are plain HIP device addresses.  All work is queued on the
+ * calling thread's library stream (simint_hip_stream); call
+ * simint_hip_synchronize before reading results. */
+#pragma once
+
+#include "simint/shell/shell.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Shell pairs of one (am1, am2) resident in device memory, in the order of
+ * the source structure (its first nshell12_clip pairs). */
+typedef struct simint_hip_pairs simint_hip_pairs;
+
+int  simint_hip_pairs_create(struct simint_multi_shellpair const * P, simint_hip_pairs ** out);
+void simint_hip_pairs_free(simint_hip_pairs * p);
+int  simint_hip_pairs_count(simint_hip_pairs const * p);
+
+/* (ab|cd) for bra pairs b = b0 .. b0+nbra-1 of P and, for each, ket pairs
+ * k = 0 .. nket[b-b0]-1 of Q: the per-bra prefix a Schwarz screen over
+ * bound-sorted pairs produces.  Quartet (b, k) is written to
+ *   d_out + out_off[b-b0] + k * ncart(abcd)       (doubles, Psi4 order).
+ * nket and out_off are host arrays.  screen_tol as in simint_compute_eri.
+ * Returns the number of quartets queued, or a negative SIMINT_ERROR_*. */
+long long simint_hip_compute_eri_prefix(simint_hip_pairs const * P, simint_hip_pairs const * Q,
+                                        int b0, int nbra, int const * nket,
+                                        long long const * out_off,
+                                        double screen_tol, double * d_out);
+
+/* (bb|bb) for every pair b of P, at d_out + b * ncart(abab). */
+long long simint_hip_compute_eri_diag(simint_hip_pairs const * P, double * d_out);
+
+int   simint_hip_synchronize(void);
+void *simint_hip_stream(void);                /* hipStream_t of the calling thread */
+const char *simint_hip_last_error(void);
+
+/* Kernel family used for a class: 1 = lane-per-quartet (registers),
+ * 2 = wave-per-quartet (LDS-staged), 0 = class not built.  The setter lets a
+ * caller (the autotuner, tests) pick a family; it returns the previous one or
+ * a negative value if that family is not built for the class. */
+int simint_hip_class_kernel(int la, int lb, int lc, int ld);
+int simint_hip_set_class_kernel(int la, int lb, int lc, int ld, int family);
+
+#ifdef __cplusplus
+}
+#endif

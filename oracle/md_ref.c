@@ -1,0 +1,272 @@
+/*
+ * md_ref.c -- independent brute-force ERI reference in long double.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.c).  This pins the Obara-Saika
+ * restatement in oracle.c with a different algorithm: McMurchie-Davidson
+ * Hermite expansion (Helgaker, Jorgensen, Olsen, "Molecular
+ * Electronic-Structure Theory", eqs. 9.5.6-9.5.7 for E, 9.9.18-9.9.20 for R,
+ * 9.9.33 for the ERI), its own Boys function (positive series below x = 50,
+ * asymptotic above), no triangle/same-shell shortcuts, every primitive
+ * combination summed, every value in long double.
+ *
+ * Integral definition (the one simint computes, skel/simint/shell/shell.c:493-498
+ * and generator/ostei/OSTEI_Writer.cpp:695-702): unnormalised Cartesian
+ * Gaussians x^i y^j z^k exp(-a r^2) contracted with the caller's coefficients
+ * (after simint_normalize_shells), output in Psi4 Cartesian order,
+ * out[((a nb + b) nc + c) nd + d].
+ *
+ * First derivatives are formed by Richardson-extrapolated central finite
+ * differences of the long-double integral with respect to the centre
+ * coordinates, so they share nothing with the analytic (a+1)/(a-1) formula.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MD_MAXL 8
+#define MD_MAXN (4 * MD_MAXL + 2)
+
+typedef long double ld;
+
+static ld md_boys(int n, ld x)
+{
+    if (x > 50.0L) {
+        /* Gamma(n+1/2) / (2 x^(n+1/2)); e^-x < 2e-22 */
+        ld g = sqrtl(3.14159265358979323846264338327950288L);
+        for (int k = 0; k < n; k++) g *= (k + 0.5L);
+        return g / (2.0L * powl(x, n + 0.5L));
+    }
+    ld term = 1.0L / (2 * n + 1), sum = term;
+    for (int k = 1; k < 4000; k++) {
+        term *= 2.0L * x / (2 * n + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-24L) break;
+    }
+    return expl(-x) * sum;
+}
+
+/* Hermite expansion coefficients E^{ij}_t (t = 0..i+j) for one direction */
+static void hermite_E(int imax, int jmax, ld p, ld XPA, ld XPB, ld E[MD_MAXL + 2][MD_MAXL + 2][2 * MD_MAXL + 5])
+{
+    memset(E, 0, sizeof(ld) * (MD_MAXL + 2) * (MD_MAXL + 2) * (2 * MD_MAXL + 5));
+    const ld o2p = 1.0L / (2.0L * p);
+    E[0][0][0] = 1.0L;
+    for (int i = 0; i <= imax; i++)
+        for (int j = 0; j <= jmax; j++) {
+            if (i == 0 && j == 0) continue;
+            for (int t = 0; t <= i + j; t++) {
+                ld v;
+                if (i > 0) {
+                    v = XPA * E[i - 1][j][t];
+                    if (t > 0) v += o2p * E[i - 1][j][t - 1];
+                    if (t + 1 <= i - 1 + j) v += (t + 1) * E[i - 1][j][t + 1];
+                } else {
+                    v = XPB * E[i][j - 1][t];
+                    if (t > 0) v += o2p * E[i][j - 1][t - 1];
+                    if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+                }
+                E[i][j][t] = v;
+            }
+        }
+}
+
+static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+static void cart_list(int l, int (*e)[3])
+{
+    int n = 0;
+    for (int ix = l; ix >= 0; ix--)
+        for (int iy = l - ix; iy >= 0; iy--) { e[n][0] = ix; e[n][1] = iy; e[n][2] = l - ix - iy; n++; }
+}
+
+typedef struct {
+    int am, nprim;
+    ld c[3];
+    const double *alpha, *coef;
+} md_shell;
+
+/* contracted (ab|cd) for one quartet, accumulated into out (long double) */
+static void md_quartet(const md_shell *S[4], ld *out)
+{
+    const int la = S[0]->am, lb = S[1]->am, lc = S[2]->am, ld_ = S[3]->am;
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld_);
+    int ea[MD_MAXL * 8][3], eb[MD_MAXL * 8][3], ec[MD_MAXL * 8][3], ed[MD_MAXL * 8][3];
+    cart_list(la, ea); cart_list(lb, eb); cart_list(lc, ec); cart_list(ld_, ed);
+    const int N = la + lb + lc + ld_;
+    const int nall = na * nb * nc * nd;
+    for (int k = 0; k < nall; k++) out[k] = 0.0L;
+
+    static __thread ld Eab[3][MD_MAXL + 2][MD_MAXL + 2][2 * MD_MAXL + 5];
+    static __thread ld Ecd[3][MD_MAXL + 2][MD_MAXL + 2][2 * MD_MAXL + 5];
+    const int D = N + 1;
+    ld *Rn = (ld *)malloc(sizeof(ld) * D * D * D);
+    ld *Rn1 = (ld *)malloc(sizeof(ld) * D * D * D);
+#define RI(t, u, v) (((t) * D + (u)) * D + (v))
+
+    const ld pi = 3.14159265358979323846264338327950288L;
+    for (int i = 0; i < S[0]->nprim; i++)
+        for (int j = 0; j < S[1]->nprim; j++)
+            for (int k = 0; k < S[2]->nprim; k++)
+                for (int l = 0; l < S[3]->nprim; l++) {
+                    const ld a = S[0]->alpha[i], b = S[1]->alpha[j], c = S[2]->alpha[k], d = S[3]->alpha[l];
+                    const ld p = a + b, q = c + d;
+                    ld P[3], Q[3], AB2 = 0, CD2 = 0, RPQ2 = 0;
+                    for (int x = 0; x < 3; x++) {
+                        P[x] = (a * S[0]->c[x] + b * S[1]->c[x]) / p;
+                        Q[x] = (c * S[2]->c[x] + d * S[3]->c[x]) / q;
+                        AB2 += (S[0]->c[x] - S[1]->c[x]) * (S[0]->c[x] - S[1]->c[x]);
+                        CD2 += (S[2]->c[x] - S[3]->c[x]) * (S[2]->c[x] - S[3]->c[x]);
+                        RPQ2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+                    }
+                    const ld alpha = p * q / (p + q);
+                    const ld cc = (ld)S[0]->coef[i] * S[1]->coef[j] * S[2]->coef[k] * S[3]->coef[l];
+                    const ld pref = cc * 2.0L * powl(pi, 2.5L) / (p * q * sqrtl(p + q))
+                                    * expl(-a * b / p * AB2) * expl(-c * d / q * CD2);
+                    for (int x = 0; x < 3; x++) {
+                        hermite_E(la, lb, p, P[x] - S[0]->c[x], P[x] - S[1]->c[x], Eab[x]);
+                        hermite_E(lc, ld_, q, Q[x] - S[2]->c[x], Q[x] - S[3]->c[x], Ecd[x]);
+                    }
+                    /* R^n_{tuv}, n from N down to 0 */
+                    const ld T = alpha * RPQ2;
+                    for (int n = N; n >= 0; n--) {
+                        const int lim = N - n;
+                        ld f = md_boys(n, T);
+                        ld m2a = 1.0L;
+                        for (int z = 0; z < n; z++) m2a *= -2.0L * alpha;
+                        for (int t = 0; t <= lim; t++)
+                            for (int u = 0; u + t <= lim; u++)
+                                for (int v = 0; v + u + t <= lim; v++) {
+                                    ld r;
+                                    if (t > 0) {
+                                        r = (P[0] - Q[0]) * Rn1[RI(t - 1, u, v)];
+                                        if (t > 1) r += (t - 1) * Rn1[RI(t - 2, u, v)];
+                                    } else if (u > 0) {
+                                        r = (P[1] - Q[1]) * Rn1[RI(t, u - 1, v)];
+                                        if (u > 1) r += (u - 1) * Rn1[RI(t, u - 2, v)];
+                                    } else if (v > 0) {
+                                        r = (P[2] - Q[2]) * Rn1[RI(t, u, v - 1)];
+                                        if (v > 1) r += (v - 1) * Rn1[RI(t, u, v - 2)];
+                                    } else {
+                                        r = m2a * f;
+                                    }
+                                    Rn[RI(t, u, v)] = r;
+                                }
+                        ld *tmp = Rn; Rn = Rn1; Rn1 = tmp;
+                    }
+                    /* now Rn1 holds level 0 */
+                    const ld *R = Rn1;
+                    int idx = 0;
+                    for (int ia = 0; ia < na; ia++)
+                        for (int ib = 0; ib < nb; ib++)
+                            for (int ic = 0; ic < nc; ic++)
+                                for (int id = 0; id < nd; id++, idx++) {
+                                    ld s = 0.0L;
+                                    const int tx = ea[ia][0] + eb[ib][0], ty = ea[ia][1] + eb[ib][1], tz = ea[ia][2] + eb[ib][2];
+                                    const int ux = ec[ic][0] + ed[id][0], uy = ec[ic][1] + ed[id][1], uz = ec[ic][2] + ed[id][2];
+                                    for (int t = 0; t <= tx; t++) {
+                                        const ld e1 = Eab[0][ea[ia][0]][eb[ib][0]][t];
+                                        if (e1 == 0.0L) continue;
+                                        for (int u = 0; u <= ty; u++) {
+                                            const ld e2 = e1 * Eab[1][ea[ia][1]][eb[ib][1]][u];
+                                            if (e2 == 0.0L) continue;
+                                            for (int v = 0; v <= tz; v++) {
+                                                const ld e3 = e2 * Eab[2][ea[ia][2]][eb[ib][2]][v];
+                                                if (e3 == 0.0L) continue;
+                                                ld s2 = 0.0L;
+                                                for (int tt = 0; tt <= ux; tt++)
+                                                    for (int uu = 0; uu <= uy; uu++)
+                                                        for (int vv = 0; vv <= uz; vv++) {
+                                                            const ld f = Ecd[0][ec[ic][0]][ed[id][0]][tt]
+                                                                       * Ecd[1][ec[ic][1]][ed[id][1]][uu]
+                                                                       * Ecd[2][ec[ic][2]][ed[id][2]][vv];
+                                                            const ld sg = ((tt + uu + vv) & 1) ? -1.0L : 1.0L;
+                                                            s2 += sg * f * R[RI(t + tt, u + uu, v + vv)];
+                                                        }
+                                                s += e3 * s2;
+                                            }
+                                        }
+                                    }
+                                    out[idx] += pref * s;
+                                }
+                }
+    free(Rn); free(Rn1);
+#undef RI
+}
+
+static void load(int nsh, const int *am, const int *nprim, const int *prim_off, const double *xyz,
+                 const double *alpha, const double *coef, md_shell *s)
+{
+    for (int i = 0; i < nsh; i++) {
+        s[i].am = am[i]; s[i].nprim = nprim[i];
+        for (int x = 0; x < 3; x++) s[i].c[x] = xyz[3 * i + x];
+        s[i].alpha = alpha + prim_off[i];
+        s[i].coef = coef + prim_off[i];
+    }
+}
+
+/* contracted integrals for a list of shell quartets (same layout and offsets
+ * as oracle_eri_quartets with deriv = 0) */
+int md_eri_quartets(int nsh, const int *am, const int *nprim, const int *prim_off,
+                    const double *xyz, const double *alpha, const double *coef,
+                    int nq, const int *quartets, const long long *offsets, double *out)
+{
+    md_shell *s = (md_shell *)malloc(sizeof(md_shell) * (size_t)nsh);
+    load(nsh, am, nprim, prim_off, xyz, alpha, coef, s);
+    for (int n = 0; n < nq; n++) {
+        const int *q = quartets + 4 * n;
+        const md_shell *S[4] = {&s[q[0]], &s[q[1]], &s[q[2]], &s[q[3]]};
+        if (S[0]->am > MD_MAXL || S[1]->am > MD_MAXL || S[2]->am > MD_MAXL || S[3]->am > MD_MAXL) {
+            free(s);
+            return -1;
+        }
+        const int nall = ncart(S[0]->am) * ncart(S[1]->am) * ncart(S[2]->am) * ncart(S[3]->am);
+        ld *buf = (ld *)malloc(sizeof(ld) * (size_t)nall);
+        md_quartet(S, buf);
+        for (int k = 0; k < nall; k++) out[offsets[n] + k] = (double)buf[k];
+        free(buf);
+    }
+    free(s);
+    return 0;
+}
+
+/* first derivatives by Richardson-extrapolated central differences:
+ * out[idx*12 + centre*3 + xyz], same layout as oracle_eri_quartets deriv=1 */
+int md_eri_deriv1_quartets(int nsh, const int *am, const int *nprim, const int *prim_off,
+                           const double *xyz, const double *alpha, const double *coef,
+                           int nq, const int *quartets, const long long *offsets, double h,
+                           double *out)
+{
+    md_shell *s = (md_shell *)malloc(sizeof(md_shell) * (size_t)nsh);
+    load(nsh, am, nprim, prim_off, xyz, alpha, coef, s);
+    for (int n = 0; n < nq; n++) {
+        const int *q = quartets + 4 * n;
+        md_shell sh[4];
+        for (int c = 0; c < 4; c++) sh[c] = s[q[c]];
+        const int nall = ncart(sh[0].am) * ncart(sh[1].am) * ncart(sh[2].am) * ncart(sh[3].am);
+        ld *p1 = (ld *)malloc(sizeof(ld) * nall), *m1 = (ld *)malloc(sizeof(ld) * nall);
+        ld *p2 = (ld *)malloc(sizeof(ld) * nall), *m2 = (ld *)malloc(sizeof(ld) * nall);
+        for (int cen = 0; cen < 4; cen++)
+            for (int x = 0; x < 3; x++) {
+                /* the same shell may sit on several centres: move every
+                 * occurrence of this centre's shell index only once per
+                 * derivative, i.e. move the centre, not the shell object */
+                const ld c0 = sh[cen].c[x];
+                const md_shell *S[4] = {&sh[0], &sh[1], &sh[2], &sh[3]};
+                sh[cen].c[x] = c0 + h;       md_quartet(S, p1);
+                sh[cen].c[x] = c0 - h;       md_quartet(S, m1);
+                sh[cen].c[x] = c0 + 0.5L * h; md_quartet(S, p2);
+                sh[cen].c[x] = c0 - 0.5L * h; md_quartet(S, m2);
+                sh[cen].c[x] = c0;
+                for (int k = 0; k < nall; k++) {
+                    const ld d1 = (p1[k] - m1[k]) / (2.0L * h);
+                    const ld d2 = (p2[k] - m2[k]) / (1.0L * h);
+                    out[offsets[n] + (long long)k * 12 + cen * 3 + x] = (double)((4.0L * d2 - d1) / 3.0L);
+                }
+            }
+        free(p1); free(m1); free(p2); free(m2);
+    }
+    free(s);
+    return 0;
+}
+
+double md_boys_value(int n, double x) { return (double)md_boys(n, (ld)x); }

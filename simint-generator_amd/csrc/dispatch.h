@@ -1,0 +1,10 @@
+// Class dispatch: (la lb | lc ld) -> kernel launcher.  The tables are
+// generated (gen/dispatch.cpp): one lane-per-quartet and one wave-per-quartet
+// launcher per class where built, and the family in use for each class.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "device_types.h"
+
+typedef hipError_t (*simint_launch_fn)(const EriArgs *, hipStream_t);
+
+simint_launch_fn simint_dispatch(int la, int lb, int lc, int ld);

@@ -1,0 +1,624 @@
+// Host runtime of the MI355X ERI engine: device set-up, shell-pair upload,
+// work lists, class dispatch, and the simint_compute_eri drop-in.
+//
+// simint_compute_eri keeps the reference's dispatch structure
+// (skel/simint/ostei/ostei.c:10-42): a shell-level screen, then the class
+// kernel for (P.am1 P.am2 | Q.am1 Q.am2).  The class kernel runs on the GPU;
+// there is no CPU path.  If the library finds no GPU every compute call
+// returns SIMINT_ERROR_NO_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "simint/simint.h"
+#include "simint/simint_hip.h"
+#include "simint/ostei/ostei.h"
+#include "device_types.h"
+#include "dispatch.h"
+
+namespace {
+
+const double kBoysGrid[366 * 32] = {
+#include "gen/boys_grid.inc"
+};
+
+inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+struct DeviceState {
+    std::once_flag once;
+    bool ok = false;
+    int id = 0;
+    double *boys = nullptr;
+};
+DeviceState g_dev;
+
+thread_local std::string t_err;
+
+void set_err(const char *what, hipError_t e)
+{
+    char buf[512];
+    snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+    t_err = buf;
+}
+
+void device_init()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        t_err = "no HIP device";
+        return;
+    }
+    int id = 0;
+    if (const char *s = getenv("SIMINT_DEVICE")) id = atoi(s);
+    if (id < 0 || id >= n) id = 0;
+    if (hipSetDevice(id) != hipSuccess) {
+        t_err = "hipSetDevice failed";
+        return;
+    }
+    double *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(kBoysGrid));
+    if (e != hipSuccess) { set_err("hipMalloc(boys)", e); return; }
+    e = hipMemcpy(d, kBoysGrid, sizeof(kBoysGrid), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { set_err("hipMemcpy(boys)", e); return; }
+    g_dev.id = id;
+    g_dev.boys = d;
+    g_dev.ok = true;
+}
+
+bool ensure_device()
+{
+    std::call_once(g_dev.once, device_init);
+    if (g_dev.ok) hipSetDevice(g_dev.id);
+    return g_dev.ok;
+}
+
+// device memory grown on demand, never shrunk while the thread lives
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= cap) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max(n, (size_t)1 << 20);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    ~DevBuf()
+    {
+        if (p) hipFree(p);
+    }
+};
+
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= cap) return hipSuccess;
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max(n, (size_t)1 << 20);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    ~PinBuf()
+    {
+        if (p) hipHostFree(p);
+    }
+};
+
+struct ThreadCtx {
+    hipStream_t s = nullptr;
+    DevBuf in, out, items;
+    PinBuf stage;
+    ThreadCtx() { hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+    ~ThreadCtx()
+    {
+        if (s) hipStreamDestroy(s);
+    }
+};
+
+ThreadCtx &ctx()
+{
+    thread_local std::unique_ptr<ThreadCtx> c;
+    if (!c) c.reset(new ThreadCtx());
+    return *c;
+}
+
+// ---------------------------------------------------------------- packing
+// A host-side image of device arrays, laid out in one buffer (256-byte
+// aligned sub-arrays) so one copy moves everything.
+struct Image {
+    std::vector<char> buf;
+    size_t add(size_t bytes)
+    {
+        size_t off = (buf.size() + 255) & ~(size_t)255;
+        buf.resize(off + std::max(bytes, (size_t)8));
+        return off;
+    }
+    template <class T> T *at(size_t off) { return reinterpret_cast<T *>(buf.data() + off); }
+};
+
+struct LinearOffs {
+    size_t alpha, inv, prefac, x, y, z, pax, pay, paz, pbx, pby, pbz, screen, a2, b2, poff, np, abx, aby, abz;
+};
+struct TiledOffs {
+    size_t alpha, inv, prefac, x, y, z, pax, pay, paz, pbx, pby, pbz, screen, a2, b2, toff, tnp, abx, aby, abz, col;
+    int ntile;
+};
+
+// bra view: pairs [0, n) of P, primitives contiguous as in the host struct
+LinearOffs pack_linear(Image &im, const simint_multi_shellpair *P, int n)
+{
+    LinearOffs o;
+    long long nprim = 0;
+    for (int i = 0; i < n; i++) nprim += P->nprim12[i];
+    const size_t db = sizeof(double) * (size_t)nprim;
+    o.alpha = im.add(db); o.inv = im.add(db); o.prefac = im.add(db);
+    o.x = im.add(db); o.y = im.add(db); o.z = im.add(db);
+    o.pax = im.add(db); o.pay = im.add(db); o.paz = im.add(db);
+    o.pbx = im.add(db); o.pby = im.add(db); o.pbz = im.add(db);
+    o.screen = im.add(db); o.a2 = im.add(db); o.b2 = im.add(db);
+    o.poff = im.add(sizeof(int) * (size_t)n); o.np = im.add(sizeof(int) * (size_t)n);
+    o.abx = im.add(sizeof(double) * (size_t)n); o.aby = im.add(sizeof(double) * (size_t)n);
+    o.abz = im.add(sizeof(double) * (size_t)n);
+    const struct { size_t off; const double *src; } cp[] = {
+        {o.alpha, P->alpha}, {o.prefac, P->prefac}, {o.x, P->x}, {o.y, P->y}, {o.z, P->z},
+        {o.pax, P->PA_x}, {o.pay, P->PA_y}, {o.paz, P->PA_z}, {o.pbx, P->PB_x}, {o.pby, P->PB_y},
+        {o.pbz, P->PB_z}, {o.a2, P->alpha2}, {o.b2, P->beta2}};
+    for (auto &c : cp) memcpy(im.at<double>(c.off), c.src, db);
+    if (P->screen) memcpy(im.at<double>(o.screen), P->screen, db);
+    else memset(im.at<double>(o.screen), 0, db);
+    double *inv = im.at<double>(o.inv);
+    const double *al = P->alpha;
+    for (long long k = 0; k < nprim; k++) inv[k] = 1.0 / al[k];
+    int *poff = im.at<int>(o.poff), *np = im.at<int>(o.np);
+    int acc = 0;
+    for (int i = 0; i < n; i++) {
+        poff[i] = acc;
+        np[i] = P->nprim12[i];
+        acc += P->nprim12[i];
+        im.at<double>(o.abx)[i] = P->AB_x[i];
+        im.at<double>(o.aby)[i] = P->AB_y[i];
+        im.at<double>(o.abz)[i] = P->AB_z[i];
+    }
+    return o;
+}
+
+// ket view: pairs [0, n) of Q in tiles of 64.  `order` lists the pairs in
+// tile order; col[] maps a tile lane back to its position in `colmap`.
+TiledOffs pack_tiled(Image &im, const simint_multi_shellpair *Q, const std::vector<int> &order,
+                     const std::vector<long long> &colmap)
+{
+    TiledOffs o;
+    const int n = (int)order.size();
+    const int ntile = (n + SIMINT_TILE - 1) / SIMINT_TILE;
+    o.ntile = ntile;
+    std::vector<long long> pstart(Q->nshell12 + 1, 0);
+    for (int i = 0; i < Q->nshell12; i++) pstart[i + 1] = pstart[i] + Q->nprim12[i];
+    std::vector<int> tnp(ntile, 0);
+    std::vector<long long> toff(ntile, 0);
+    long long total = 0;
+    for (int t = 0; t < ntile; t++) {
+        int mx = 0;
+        for (int l = 0; l < SIMINT_TILE && t * SIMINT_TILE + l < n; l++)
+            mx = std::max(mx, Q->nprim12[order[t * SIMINT_TILE + l]]);
+        tnp[t] = mx;
+        toff[t] = total;
+        total += (long long)mx * SIMINT_TILE;
+    }
+    const size_t db = sizeof(double) * (size_t)std::max(total, 1LL);
+    o.alpha = im.add(db); o.inv = im.add(db); o.prefac = im.add(db);
+    o.x = im.add(db); o.y = im.add(db); o.z = im.add(db);
+    o.pax = im.add(db); o.pay = im.add(db); o.paz = im.add(db);
+    o.pbx = im.add(db); o.pby = im.add(db); o.pbz = im.add(db);
+    o.screen = im.add(db); o.a2 = im.add(db); o.b2 = im.add(db);
+    o.toff = im.add(sizeof(long long) * (size_t)ntile);
+    o.tnp = im.add(sizeof(int) * (size_t)ntile);
+    const size_t nl = (size_t)ntile * SIMINT_TILE;
+    o.abx = im.add(sizeof(double) * nl); o.aby = im.add(sizeof(double) * nl); o.abz = im.add(sizeof(double) * nl);
+    o.col = im.add(sizeof(long long) * nl);
+    double *f[15] = {im.at<double>(o.alpha), im.at<double>(o.inv), im.at<double>(o.prefac), im.at<double>(o.x),
+                     im.at<double>(o.y), im.at<double>(o.z), im.at<double>(o.pax), im.at<double>(o.pay),
+                     im.at<double>(o.paz), im.at<double>(o.pbx), im.at<double>(o.pby), im.at<double>(o.pbz),
+                     im.at<double>(o.screen), im.at<double>(o.a2), im.at<double>(o.b2)};
+    const double *src[15] = {Q->alpha, nullptr, Q->prefac, Q->x, Q->y, Q->z, Q->PA_x, Q->PA_y, Q->PA_z,
+                             Q->PB_x, Q->PB_y, Q->PB_z, Q->screen, Q->alpha2, Q->beta2};
+    // padding: alpha = 1, everything else 0 (zero contribution)
+    for (long long k = 0; k < total; k++) {
+        for (int a = 0; a < 15; a++) f[a][k] = 0.0;
+        f[0][k] = 1.0;
+        f[1][k] = 1.0;
+        f[13][k] = 1.0;
+        f[14][k] = 1.0;
+    }
+    long long *colp = im.at<long long>(o.col);
+    for (int t = 0; t < ntile; t++) {
+        im.at<long long>(o.toff)[t] = toff[t];
+        im.at<int>(o.tnp)[t] = tnp[t];
+        for (int l = 0; l < SIMINT_TILE; l++) {
+            const size_t tl = (size_t)t * SIMINT_TILE + l;
+            const int pos = t * SIMINT_TILE + l;
+            if (pos >= n) {
+                colp[tl] = -1;
+                im.at<double>(o.abx)[tl] = im.at<double>(o.aby)[tl] = im.at<double>(o.abz)[tl] = 0.0;
+                continue;
+            }
+            const int q = order[pos];
+            colp[tl] = colmap[pos];
+            im.at<double>(o.abx)[tl] = Q->AB_x[q];
+            im.at<double>(o.aby)[tl] = Q->AB_y[q];
+            im.at<double>(o.abz)[tl] = Q->AB_z[q];
+            for (int j = 0; j < Q->nprim12[q]; j++) {
+                const long long k = toff[t] + (long long)j * SIMINT_TILE + l;
+                const long long s = pstart[q] + j;
+                for (int a = 0; a < 15; a++) {
+                    if (a == 1) f[a][k] = 1.0 / Q->alpha[s];
+                    else if (src[a]) f[a][k] = src[a][s];
+                    else f[a][k] = 0.0;
+                }
+            }
+        }
+    }
+    return o;
+}
+
+PairsLinear linear_ptrs(char *base, const LinearOffs &o)
+{
+    PairsLinear p;
+    auto D = [&](size_t off) { return reinterpret_cast<const double *>(base + off); };
+    p.alpha = D(o.alpha); p.inv_alpha = D(o.inv); p.prefac = D(o.prefac);
+    p.x = D(o.x); p.y = D(o.y); p.z = D(o.z);
+    p.pax = D(o.pax); p.pay = D(o.pay); p.paz = D(o.paz);
+    p.pbx = D(o.pbx); p.pby = D(o.pby); p.pbz = D(o.pbz);
+    p.screen = D(o.screen); p.alpha2 = D(o.a2); p.beta2 = D(o.b2);
+    p.poff = reinterpret_cast<const int *>(base + o.poff);
+    p.np = reinterpret_cast<const int *>(base + o.np);
+    p.abx = D(o.abx); p.aby = D(o.aby); p.abz = D(o.abz);
+    return p;
+}
+
+PairsTiled tiled_ptrs(char *base, const TiledOffs &o)
+{
+    PairsTiled p;
+    auto D = [&](size_t off) { return reinterpret_cast<const double *>(base + off); };
+    p.alpha = D(o.alpha); p.inv_alpha = D(o.inv); p.prefac = D(o.prefac);
+    p.x = D(o.x); p.y = D(o.y); p.z = D(o.z);
+    p.pax = D(o.pax); p.pay = D(o.pay); p.paz = D(o.paz);
+    p.pbx = D(o.pbx); p.pby = D(o.pby); p.pbz = D(o.pbz);
+    p.screen = D(o.screen); p.alpha2 = D(o.a2); p.beta2 = D(o.b2);
+    p.toff = reinterpret_cast<const long long *>(base + o.toff);
+    p.tnp = reinterpret_cast<const int *>(base + o.tnp);
+    p.abx = D(o.abx); p.aby = D(o.aby); p.abz = D(o.abz);
+    p.col = reinterpret_cast<const long long *>(base + o.col);
+    return p;
+}
+
+int validate(const simint_multi_shellpair *P)
+{
+    if (!P) return SIMINT_ERROR_ARGUMENT;
+    if (P->am1 < 0 || P->am2 < 0 || P->am1 > SIMINT_OSTEI_MAXAM || P->am2 > SIMINT_OSTEI_MAXAM)
+        return SIMINT_ERROR_UNSUPPORTED;
+    if (P->nshell12_clip < 0 || P->nshell12_clip > P->nshell12) return SIMINT_ERROR_ARGUMENT;
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- C API
+extern "C" {
+
+void simint_init(void) { ensure_device(); }
+
+void simint_finalize(void)
+{
+    // device memory is released when the process ends; per-thread buffers
+    // when their thread exits
+}
+
+const char *simint_hip_last_error(void) { return t_err.c_str(); }
+
+int simint_compute_ostei(const simint_multi_shellpair *P, const simint_multi_shellpair *Q, double screen_tol,
+                         double *restrict work, double *restrict integrals)
+{
+    return simint_compute_ostei_deriv(0, P, Q, screen_tol, work, integrals);
+}
+
+int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const simint_multi_shellpair *Q,
+                               double screen_tol, double *restrict work, double *restrict integrals)
+{
+    (void)work;
+    int v = validate(P);
+    if (v) return v;
+    v = validate(Q);
+    if (v) return v;
+    if (deriv != 0) {
+        t_err = "derivative integrals are not built";
+        return SIMINT_ERROR_UNSUPPORTED;
+    }
+    // shell-level screen (ostei.c:17-20): screen values are not square-rooted
+    const double tol2 = screen_tol * screen_tol;
+    if (screen_tol > 0.0 && (P->screen_max * Q->screen_max) < tol2) return -1;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const int la = P->am1, lb = P->am2, lc = Q->am1, ld = Q->am2;
+    const simint_launch_fn fn = simint_dispatch(la, lb, lc, ld);
+    if (!fn) {
+        t_err = "class not built";
+        return SIMINT_ERROR_UNSUPPORTED;
+    }
+    const int nP = P->nshell12_clip, nQ = Q->nshell12_clip;
+    if (nP == 0 || nQ == 0) return 0;
+    const int nc = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+
+    // kets into tiles, grouped by primitive count (stable), so a tile's
+    // lanes share one primitive loop
+    std::vector<int> order(nQ);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return Q->nprim12[a] > Q->nprim12[b]; });
+    std::vector<long long> colmap(order.begin(), order.end());
+
+    Image im;
+    const LinearOffs lo = pack_linear(im, P, nP);
+    const TiledOffs to = pack_tiled(im, Q, order, colmap);
+    const size_t items_off = im.add(sizeof(WorkItem) * (size_t)nP * to.ntile);
+    WorkItem *items = im.at<WorkItem>(items_off);
+    long long nit = 0;
+    for (int b = 0; b < nP; b++)
+        for (int t = 0; t < to.ntile; t++) {
+            WorkItem &w = items[nit++];
+            w.bra = b;
+            w.tile = t;
+            w.lane_lo = 0;
+            w.lane_hi = std::min(SIMINT_TILE, nQ - t * SIMINT_TILE);
+            w.obase = (long long)b * nQ * nc;
+        }
+
+    ThreadCtx &c = ctx();
+    hipError_t e;
+    if ((e = c.stage.reserve(im.buf.size())) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
+    if ((e = c.in.reserve(im.buf.size())) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    const size_t out_bytes = sizeof(double) * (size_t)nP * nQ * nc;
+    if ((e = c.out.reserve(out_bytes)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    memcpy(c.stage.p, im.buf.data(), im.buf.size());
+    e = hipMemcpyAsync(c.in.p, c.stage.p, im.buf.size(), hipMemcpyHostToDevice, c.s);
+    if (e != hipSuccess) { set_err("hipMemcpyAsync(H2D)", e); return SIMINT_ERROR_HIP; }
+
+    char *base = static_cast<char *>(c.in.p);
+    EriArgs a;
+    a.bra = linear_ptrs(base, lo);
+    a.ket = tiled_ptrs(base, to);
+    a.items = reinterpret_cast<const WorkItem *>(base + items_off);
+    a.nitems = nit;
+    a.out = static_cast<double *>(c.out.p);
+    a.boys_grid = g_dev.boys;
+    a.tol2 = (screen_tol > 0.0 && P->screen && Q->screen) ? tol2 : 0.0;
+    a.ket_screen_max = Q->screen_max;
+    a.stride = nc;
+    if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    e = hipMemcpyAsync(integrals, c.out.p, out_bytes, hipMemcpyDeviceToHost, c.s);
+    if (e != hipSuccess) { set_err("hipMemcpyAsync(D2H)", e); return SIMINT_ERROR_HIP; }
+    e = hipStreamSynchronize(c.s);
+    if (e != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
+    return nP * nQ;
+}
+
+int simint_compute_eri(const simint_multi_shellpair *P, const simint_multi_shellpair *Q, double screen_tol,
+                       double *restrict work, double *restrict integrals)
+{
+    return simint_compute_ostei(P, Q, screen_tol, work, integrals);
+}
+
+int simint_compute_eri_deriv(int deriv, const simint_multi_shellpair *P, const simint_multi_shellpair *Q,
+                             double screen_tol, double *restrict work, double *restrict integrals)
+{
+    return simint_compute_ostei_deriv(deriv, P, Q, screen_tol, work, integrals);
+}
+
+size_t simint_eri_worksize(int derorder, int maxam) { return simint_ostei_worksize(derorder, maxam); }
+size_t simint_eri_workmem(int derorder, int maxam) { return simint_ostei_workmem(derorder, maxam); }
+
+// --------------------------------------------------------- device-resident
+struct simint_hip_pairs {
+    int am1, am2, n;
+    double screen_max;
+    void *dev = nullptr;
+    PairsLinear lin;
+    PairsTiled til;
+    int ntile;
+};
+
+int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **out)
+{
+    int v = validate(P);
+    if (v) return v;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const int n = P->nshell12_clip;
+    Image im;
+    const LinearOffs lo = pack_linear(im, P, n);
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<long long> colmap(order.begin(), order.end());
+    const TiledOffs to = pack_tiled(im, P, order, colmap);
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, im.buf.size());
+    if (e != hipSuccess) { set_err("hipMalloc(pairs)", e); return SIMINT_ERROR_HIP; }
+    e = hipMemcpy(d, im.buf.data(), im.buf.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { hipFree(d); set_err("hipMemcpy(pairs)", e); return SIMINT_ERROR_HIP; }
+    simint_hip_pairs *p = new simint_hip_pairs;
+    p->am1 = P->am1;
+    p->am2 = P->am2;
+    p->n = n;
+    p->screen_max = P->screen_max;
+    p->dev = d;
+    p->lin = linear_ptrs(static_cast<char *>(d), lo);
+    p->til = tiled_ptrs(static_cast<char *>(d), to);
+    p->ntile = to.ntile;
+    *out = p;
+    return 0;
+}
+
+void simint_hip_pairs_free(simint_hip_pairs *p)
+{
+    if (!p) return;
+    if (p->dev) hipFree(p->dev);
+    delete p;
+}
+
+int simint_hip_pairs_count(const simint_hip_pairs *p) { return p ? p->n : 0; }
+
+}  // extern "C"
+
+namespace {
+
+// items for bra b = b0 + i: its ket prefix [0, nket[i]) in tiles of 64
+__global__ void expand_prefix_items(int b0, int nbra, const int *nket, const long long *tile_start,
+                                    const long long *out_off, WorkItem *items)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbra) return;
+    const int nk = nket[i];
+    const long long s = tile_start[i];
+    const int nt = (nk + SIMINT_TILE - 1) / SIMINT_TILE;
+    for (int t = 0; t < nt; t++) {
+        WorkItem w;
+        w.bra = b0 + i;
+        w.tile = t;
+        w.lane_lo = 0;
+        w.lane_hi = min(SIMINT_TILE, nk - t * SIMINT_TILE);
+        w.obase = out_off[i];
+        items[s + t] = w;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
+                                        const int *nket, const long long *out_off, double screen_tol,
+                                        double *d_out)
+{
+    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n) return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const simint_launch_fn fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
+    if (!fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    if (nbra == 0) return 0;
+    std::vector<long long> tstart(nbra + 1, 0);
+    long long nq = 0;
+    for (int i = 0; i < nbra; i++) {
+        if (nket[i] < 0 || nket[i] > Q->n) return SIMINT_ERROR_ARGUMENT;
+        tstart[i + 1] = tstart[i] + (nket[i] + SIMINT_TILE - 1) / SIMINT_TILE;
+        nq += nket[i];
+    }
+    const long long nitems = tstart[nbra];
+    if (nitems == 0) return 0;
+    ThreadCtx &c = ctx();
+    Image im;
+    const size_t o_nk = im.add(sizeof(int) * nbra);
+    const size_t o_ts = im.add(sizeof(long long) * nbra);
+    const size_t o_oo = im.add(sizeof(long long) * nbra);
+    memcpy(im.at<int>(o_nk), nket, sizeof(int) * nbra);
+    memcpy(im.at<long long>(o_ts), tstart.data(), sizeof(long long) * nbra);
+    memcpy(im.at<long long>(o_oo), out_off, sizeof(long long) * nbra);
+    hipError_t e;
+    // the previous call's kernels may still read c.in / c.items: wait for them
+    if ((e = hipStreamSynchronize(c.s)) != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
+    if ((e = c.stage.reserve(im.buf.size())) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
+    if ((e = c.in.reserve(im.buf.size())) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    if ((e = c.items.reserve(sizeof(WorkItem) * (size_t)nitems)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    memcpy(c.stage.p, im.buf.data(), im.buf.size());
+    if ((e = hipMemcpyAsync(c.in.p, c.stage.p, im.buf.size(), hipMemcpyHostToDevice, c.s)) != hipSuccess) {
+        set_err("hipMemcpyAsync", e);
+        return SIMINT_ERROR_HIP;
+    }
+    char *base = static_cast<char *>(c.in.p);
+    WorkItem *items = static_cast<WorkItem *>(c.items.p);
+    hipLaunchKernelGGL(expand_prefix_items, dim3((nbra + 255) / 256), dim3(256), 0, c.s, b0, nbra,
+                       reinterpret_cast<const int *>(base + o_nk), reinterpret_cast<const long long *>(base + o_ts),
+                       reinterpret_cast<const long long *>(base + o_oo), items);
+    if ((e = hipGetLastError()) != hipSuccess) { set_err("expand launch", e); return SIMINT_ERROR_HIP; }
+    EriArgs a;
+    a.bra = P->lin;
+    a.ket = Q->til;
+    a.items = items;
+    a.nitems = nitems;
+    a.out = d_out;
+    a.boys_grid = g_dev.boys;
+    a.tol2 = screen_tol > 0.0 ? screen_tol * screen_tol : 0.0;
+    a.ket_screen_max = Q->screen_max;
+    a.stride = ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
+    if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    return nq;
+}
+
+long long simint_hip_compute_eri_diag(const simint_hip_pairs *P, double *d_out)
+{
+    if (!P) return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const simint_launch_fn fn = simint_dispatch(P->am1, P->am2, P->am1, P->am2);
+    if (!fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    if (P->n == 0) return 0;
+    ThreadCtx &c = ctx();
+    std::vector<WorkItem> items(P->n);
+    for (int b = 0; b < P->n; b++) {
+        items[b].bra = b;
+        items[b].tile = b / SIMINT_TILE;
+        items[b].lane_lo = b % SIMINT_TILE;
+        items[b].lane_hi = b % SIMINT_TILE + 1;
+        items[b].obase = 0;
+    }
+    hipError_t e;
+    if ((e = hipStreamSynchronize(c.s)) != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
+    const size_t bytes = sizeof(WorkItem) * items.size();
+    if ((e = c.stage.reserve(bytes)) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
+    if ((e = c.items.reserve(bytes)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    memcpy(c.stage.p, items.data(), bytes);
+    if ((e = hipMemcpyAsync(c.items.p, c.stage.p, bytes, hipMemcpyHostToDevice, c.s)) != hipSuccess) {
+        set_err("hipMemcpyAsync", e);
+        return SIMINT_ERROR_HIP;
+    }
+    EriArgs a;
+    a.bra = P->lin;
+    a.ket = P->til;
+    a.items = static_cast<const WorkItem *>(c.items.p);
+    a.nitems = P->n;
+    a.out = d_out;
+    a.boys_grid = g_dev.boys;
+    a.tol2 = 0.0;
+    a.ket_screen_max = P->screen_max;
+    a.stride = ncart(P->am1) * ncart(P->am2) * ncart(P->am1) * ncart(P->am2);
+    if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    return P->n;
+}
+
+int simint_hip_synchronize(void)
+{
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    hipError_t e = hipStreamSynchronize(ctx().s);
+    if (e != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
+    return 0;
+}
+
+void *simint_hip_stream(void)
+{
+    if (!ensure_device()) return nullptr;
+    return (void *)ctx().s;
+}
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+"""HIP writer: lane-per-quartet (LPQ) kernels for one (ab|cd) class.
+
+Mapping.  One wavefront (64 lanes) takes one work item: a bra shell pair,
+uniform across the wave, against the 64 ket shell pairs of a ket tile, one
+per lane.  Bra primitive data are wave-uniform scalar loads; ket primitive j
+of every lane is one contiguous 512-byte run per field (tiled layout, see
+csrc/device_types.h).  Each lane loops over bra x ket primitive pairs,
+evaluates the Boys function, the Obara-Saika VRR as straight-line FP64 code
+held in registers, and accumulates the contracted HRR inputs in registers;
+then it runs the bra and ket HRR in registers and writes its ncart(abcd)
+integrals in Psi4 order.
+
+The arithmetic of every step is the reference's (setup:
+generator/ostei/OSTEI_Writer.cpp:595-702; VRR: OSTEI_VRR_Writer.cpp:95-125;
+HRR: OSTEI_HRR_Writer.cpp:22-99); only the (component, m) values that feed
+the result are emitted.
+"""
+from .rrgraph import graph, carts, cart_index, ncart, I, J, K, L
+
+AMCHAR = "spdfghik"
+
+
+def class_name(am):
+    return "_".join(AMCHAR[x] for x in am)
+
+
+def _fmt_int(k):
+    return "%d.0" % k
+
+
+class _Writer:
+    def __init__(self):
+        self.lines = []
+
+    def __call__(self, s=""):
+        self.lines.append(s)
+
+    def text(self):
+        return "\n".join(self.lines) + "\n"
+
+
+def emit_body(am, w, indent="        "):
+    """Emit the per-primitive-quartet code (setup, Boys, VRR, accumulation).
+
+    Assumes in scope: P_alpha, P_inv, P_prefac, P_x/y/z, P_PA or P_PB (x,y,z),
+    Q_* likewise, grid, and accumulators acc<k>."""
+    g = graph(*am)
+    Ltot = g.Ltot
+    nodes = g.node_list
+    used_steps = set(n.step for n in nodes if n.step is not None)
+    has_bra = bool(used_steps & {I, J})
+    has_ket = bool(used_steps & {K, L})
+    w(indent + "const double PQalpha_mul = P_alpha * Q_alpha;")
+    w(indent + "const double PQalpha_sum = P_alpha + Q_alpha;")
+    w(indent + "const double one_over_PQalpha_sum = 1.0 / PQalpha_sum;")
+    w(indent + "const double PQx = P_x - Q_x;")
+    w(indent + "const double PQy = P_y - Q_y;")
+    w(indent + "const double PQz = P_z - Q_z;")
+    w(indent + "double R2 = PQx * PQx;")
+    w(indent + "R2 = fma(PQy, PQy, R2);")
+    w(indent + "R2 = fma(PQz, PQz, R2);")
+    w(indent + "const double alpha = PQalpha_mul * one_over_PQalpha_sum;")
+    w(indent + "const double F_x = R2 * alpha;")
+    w(indent + "double F[%d];" % (Ltot + 1))
+    w(indent + "boys_eval<%d>(grid, F_x, F);" % Ltot)
+    w(indent + "const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
+    # integer constants k/(2p), k/(2q), k/(2pq)
+    need = {"2p": set(), "2q": set(), "2pq": set()}
+    for n in nodes:
+        if n.step is None:
+            continue
+        bra = n.step in (I, J)
+        if n.src[2] is not None:
+            need["2p" if bra else "2q"].add(n.fac[0])
+        if n.src[4] is not None:
+            need["2p" if bra else "2q"].add(n.fac[1])
+        if n.src[6] is not None:
+            need["2pq"].add(n.fac[2])
+        if n.src[7] is not None:
+            need["2pq"].add(n.fac[3])
+    if need["2p"]:
+        w(indent + "const double one_over_2p = 0.5 * P_inv;")
+    if need["2q"]:
+        w(indent + "const double one_over_2q = 0.5 * Q_inv;")
+    if need["2pq"]:
+        w(indent + "const double one_over_2pq = 0.5 * one_over_PQalpha_sum;")
+    for kind in ("2p", "2q", "2pq"):
+        for k in sorted(need[kind]):
+            if k > 1:
+                w(indent + "const double c%d_%s = %s * one_over_%s;" % (k, kind, _fmt_int(k), kind))
+    if has_bra:
+        w(indent + "const double a_over_p = (-alpha) * P_inv;")
+        w(indent + "const double aop_PQ[3] = {a_over_p * PQx, a_over_p * PQy, a_over_p * PQz};")
+    if has_ket:
+        w(indent + "const double a_over_q_pos = alpha * Q_inv;")
+        w(indent + "const double aoq_PQ[3] = {a_over_q_pos * PQx, a_over_q_pos * PQy, a_over_q_pos * PQz};")
+        w(indent + "const double a_over_q = -a_over_q_pos;")
+
+    def cname(k, kind):
+        return ("one_over_%s" % kind) if k == 1 else ("c%d_%s" % (k, kind))
+
+    idx = {}
+    for n in nodes:
+        key = (n.q, n.m)
+        v = "v%d" % len(idx)
+        idx[key] = v
+        if n.step is None:
+            w(indent + "const double %s = F[%d] * prefac;" % (v, n.m))
+            continue
+        d = n.dir
+        bra = n.step in (I, J)
+        if n.step == I:
+            pa = "P_PA[%d]" % d
+        elif n.step == J:
+            pa = "P_PB[%d]" % d
+        elif n.step == K:
+            pa = "Q_PA[%d]" % d
+        else:
+            pa = "Q_PB[%d]" % d
+        aop = ("aop_PQ[%d]" if bra else "aoq_PQ[%d]") % d
+        aover = "a_over_p" if bra else "a_over_q"
+        k2 = "2p" if bra else "2q"
+        s = [idx[x] if x is not None else None for x in n.src]
+        expr = "%s * %s" % (pa, s[0])
+        expr = "fma(%s, %s, %s)" % (aop, s[1], expr)
+        if s[2] is not None:
+            expr = "fma(%s, fma(%s, %s, %s), %s)" % (cname(n.fac[0], k2), aover, s[3], s[2], expr)
+        if s[4] is not None:
+            expr = "fma(%s, fma(%s, %s, %s), %s)" % (cname(n.fac[1], k2), aover, s[5], s[4], expr)
+        if s[6] is not None:
+            expr = "fma(%s, %s, %s)" % (cname(n.fac[2], "2pq"), s[6], expr)
+        if s[7] is not None:
+            expr = "fma(%s, %s, %s)" % (cname(n.fac[3], "2pq"), s[7], expr)
+        w(indent + "const double %s = %s;" % (v, expr))
+    for t, key in enumerate(g.top_keys):
+        w(indent + "acc%d += %s;" % (t, idx[key]))
+
+
+def emit_hrr(am, w, indent="    ", out="o", acc="acc"):
+    """Emit the bra then ket HRR on acc<k>, storing into out[k] (Psi4 order)."""
+    g = graph(*am)
+    la, lb, lc, ld = am
+    bra_ops, bra_map, ket_ops, ket_map = g.hrr_program()
+    ket_tops = g.ket_tops()
+    bra_tops = g.bra_tops()
+    nf = len(ket_tops)
+    etop = {e: i for i, e in enumerate(bra_tops)}
+    ftop = {f: i for i, f in enumerate(ket_tops)}
+
+    def topvar(e, f):
+        return "%s%d" % (acc, etop[e] * nf + ftop[f])
+
+    # bra HRR for every top ket component f -> names hb_<f>_<a>_<b>
+    names = {}
+    cnt = [0]
+
+    def newname(prefix):
+        cnt[0] += 1
+        return "%s%d" % (prefix, cnt[0])
+
+    def bra_val(src, f):
+        if src[0] == "top":
+            return topvar(src[1], f)
+        return names[("b", f, src[1], src[2])]
+
+    for f in ket_tops:
+        for (dst, s0, s1, sign, d) in bra_ops:
+            nm = newname("hb")
+            names[("b", f, dst[1], dst[2])] = nm
+            ab = ("hAB[%d]" % d) if sign > 0 else ("(-hAB[%d])" % d)
+            w(indent + "const double %s = fma(%s, %s, %s);" % (nm, ab, bra_val(s1, f), bra_val(s0, f)))
+
+    # ket HRR for every final bra component
+    nb, nc, nd = ncart(lb), ncart(lc), ncart(ld)
+    for ia, a in enumerate(carts(la)):
+        for ib, b in enumerate(carts(lb)):
+            bm = bra_map[(a, b)]
+
+            def abf(fc):
+                return bra_val(bm, fc)
+
+            def ket_val(src):
+                if src[0] == "top":
+                    return abf(src[1])
+                return names[("k", a, b, src[1], src[2])]
+
+            for (dst, s0, s1, sign, d) in ket_ops:
+                nm = newname("hk")
+                names[("k", a, b, dst[1], dst[2])] = nm
+                cd = ("hCD[%d]" % d) if sign > 0 else ("(-hCD[%d])" % d)
+                w(indent + "const double %s = fma(%s, %s, %s);" % (nm, cd, ket_val(s1), ket_val(s0)))
+            for ic, c in enumerate(carts(lc)):
+                for idd, dd in enumerate(carts(ld)):
+                    k = ((ia * nb + ib) * nc + ic) * nd + idd
+                    w(indent + "%s[%d] = %s;" % (out, k, ket_val(ket_map[(c, dd)])))
+
+
+def emit_lpq(am):
+    g = graph(*am)
+    la, lb, lc, ld = am
+    name = class_name(am)
+    ntop = len(g.top_keys)
+    w = _Writer()
+    w("// Generated by simint-generator_amd/emitter/hip_lpq.py -- class (%s)" % name.replace("_", ""))
+    w('#include "../device_types.h"')
+    w('#include "../boys_device.h"')
+    w("")
+    w("__global__ void __launch_bounds__(64) eri_lpq_%s(const EriArgs A)" % name)
+    w("{")
+    w("    const WorkItem wi = A.items[blockIdx.x];")
+    w("    const int lane = threadIdx.x;")
+    w("    const double *__restrict__ grid = A.boys_grid;")
+    w("    const int ib0 = A.bra.poff[wi.bra];")
+    w("    const int ib1 = ib0 + A.bra.np[wi.bra];")
+    w("    const int nq = A.ket.tnp[wi.tile];")
+    w("    const long long tl = (long long)wi.tile * SIMINT_TILE + lane;")
+    w("    const long long kb = A.ket.toff[wi.tile] + lane;")
+    w("    const bool screen = A.tol2 > 0.0;")
+    for t in range(ntop):
+        w("    double acc%d = 0.0;" % t)
+    bra_pa = "pa" if g.bra_j else "pb"
+    ket_pa = "pa" if g.ket_l else "pb"
+    w("    for (int i = ib0; i < ib1; ++i) {")
+    w("        const double P_alpha = A.bra.alpha[i];")
+    w("        const double P_inv = A.bra.inv_alpha[i];")
+    w("        const double P_prefac = A.bra.prefac[i];")
+    w("        const double P_x = A.bra.x[i], P_y = A.bra.y[i], P_z = A.bra.z[i];")
+    w("        const double P_%s[3] = {A.bra.%sx[i], A.bra.%sy[i], A.bra.%sz[i]};"
+      % ("PA" if g.bra_j else "PB", bra_pa, bra_pa, bra_pa))
+    w("        const double P_screen = screen ? A.bra.screen[i] : 0.0;")
+    w("        if (screen && P_screen * A.ket_screen_max < A.tol2) continue;")
+    w("        for (int j = 0; j < nq; ++j) {")
+    w("            const long long kj = kb + (long long)SIMINT_TILE * j;")
+    w("            if (screen && A.ket.screen[kj] * P_screen < A.tol2) continue;")
+    w("            const double Q_alpha = A.ket.alpha[kj];")
+    w("            const double Q_inv = A.ket.inv_alpha[kj];")
+    w("            const double Q_prefac = A.ket.prefac[kj];")
+    w("            const double Q_x = A.ket.x[kj], Q_y = A.ket.y[kj], Q_z = A.ket.z[kj];")
+    w("            const double Q_%s[3] = {A.ket.%sx[kj], A.ket.%sy[kj], A.ket.%sz[kj]};"
+      % ("PA" if g.ket_l else "PB", ket_pa, ket_pa, ket_pa))
+    emit_body(am, w, indent="            ")
+    w("        }")
+    w("    }")
+    w("    if (lane < wi.lane_lo || lane >= wi.lane_hi) return;")
+    w("    const long long colv = A.ket.col[tl];")
+    w("    double *__restrict__ o = A.out + wi.obase + colv * A.stride;")
+    if g.bra_hrr_needed():
+        w("    const double hAB[3] = {A.bra.abx[wi.bra], A.bra.aby[wi.bra], A.bra.abz[wi.bra]};")
+    if g.ket_hrr_needed():
+        w("    const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
+    emit_hrr(am, w)
+    w("}")
+    w("")
+    w('extern "C" hipError_t simint_launch_lpq_%s(const EriArgs *a, hipStream_t s)' % name)
+    w("{")
+    w("    if (a->nitems <= 0) return hipSuccess;")
+    w("    hipLaunchKernelGGL(eri_lpq_%s, dim3((unsigned)a->nitems), dim3(64), 0, s, *a);" % name)
+    w("    return hipGetLastError();")
+    w("}")
+    return w.text()

@@ -1,0 +1,246 @@
+"""ctypes mirror of the simint C API exported by libsimint.so.
+
+Python-side helpers for tests, the benchmark and the autotuner.  Struct
+layouts match include/simint/shell/shell.h (the reference's
+skel/simint/shell/shell.h:18-85 with alpha2/beta2 present).  Loading fails
+loudly if the library has not been built: there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsimint.so")
+
+c_int, c_double, c_size_t, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
+dptr = ctypes.POINTER(ctypes.c_double)
+iptr = ctypes.POINTER(ctypes.c_int)
+llptr = ctypes.POINTER(ctypes.c_longlong)
+
+SCREEN_NONE, SCREEN_SCHWARZ, SCREEN_FASTSCHWARZ = 0, 1, 2
+ERROR_NO_DEVICE, ERROR_UNSUPPORTED, ERROR_HIP, ERROR_ARGUMENT = -2, -3, -4, -5
+
+
+class Shell(ctypes.Structure):
+    _fields_ = [("am", c_int), ("nprim", c_int), ("x", c_double), ("y", c_double), ("z", c_double),
+                ("alpha", dptr), ("coef", dptr), ("memsize", c_size_t), ("ptr", c_void_p)]
+
+
+class MultiShellpair(ctypes.Structure):
+    _fields_ = [("am1", c_int), ("am2", c_int), ("nprim", c_int), ("nshell12", c_int), ("nshell12_clip", c_int),
+                ("nprim12", iptr),
+                ("AB_x", dptr), ("AB_y", dptr), ("AB_z", dptr),
+                ("x", dptr), ("y", dptr), ("z", dptr),
+                ("PA_x", dptr), ("PA_y", dptr), ("PA_z", dptr),
+                ("PB_x", dptr), ("PB_y", dptr), ("PB_z", dptr),
+                ("alpha", dptr), ("alpha2", dptr), ("beta2", dptr),
+                ("prefac", dptr), ("screen", dptr), ("screen_max", c_double),
+                ("memsize", c_size_t), ("ptr", c_void_p)]
+
+
+# every exported symbol of include/simint/*.h
+EXPORTS = [
+    "simint_init", "simint_finalize",
+    "simint_compute_eri", "simint_compute_eri_deriv", "simint_eri_worksize", "simint_eri_workmem",
+    "simint_compute_ostei", "simint_compute_ostei_deriv",
+    "simint_initialize_shell", "simint_initialize_shells", "simint_allocate_shell", "simint_free_shell",
+    "simint_free_shells", "simint_copy_shell", "simint_normalize_shells", "simint_create_shell",
+    "simint_create_zero_shell", "simint_initialize_multi_shellpair", "simint_initialize_multi_shellpairs",
+    "simint_allocate_multi_shellpair", "simint_allocate_multi_shellpair2", "simint_free_multi_shellpair",
+    "simint_free_multi_shellpairs", "simint_fill_multi_shellpair", "simint_fill_multi_shellpair2",
+    "simint_create_multi_shellpair", "simint_create_multi_shellpair2", "simint_cat_multi_shellpair",
+    "simint_cat_shellpairs",
+    "simint_shellscreen", "simint_primscreen", "simint_shellscreen_schwarz", "simint_shellscreen_fastschwarz",
+    "simint_primscreen_schwarz", "simint_primscreen_fastschwarz",
+    "simint_hip_pairs_create", "simint_hip_pairs_free", "simint_hip_pairs_count",
+    "simint_hip_compute_eri_prefix", "simint_hip_compute_eri_diag", "simint_hip_synchronize",
+    "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
+]
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsimint.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        SP, MP = ctypes.POINTER(Shell), ctypes.POINTER(MultiShellpair)
+        L.simint_create_shell.argtypes = [c_int, c_int, c_double, c_double, c_double, dptr, dptr, SP]
+        L.simint_normalize_shells.argtypes = [c_int, SP]
+        L.simint_free_shell.argtypes = [SP]
+        L.simint_initialize_shell.argtypes = [SP]
+        L.simint_copy_shell.argtypes = [SP, SP]
+        L.simint_initialize_multi_shellpair.argtypes = [MP]
+        L.simint_free_multi_shellpair.argtypes = [MP]
+        L.simint_create_multi_shellpair.argtypes = [c_int, SP, c_int, SP, MP, c_int]
+        L.simint_create_multi_shellpair2.argtypes = [c_int, SP, MP, c_int]
+        L.simint_cat_shellpairs.argtypes = [c_int, ctypes.POINTER(MP), MP, c_int]
+        L.simint_cat_multi_shellpair.argtypes = [c_int, ctypes.POINTER(MP), MP, c_int]
+        L.simint_compute_eri.argtypes = [MP, MP, c_double, dptr, dptr]
+        L.simint_compute_eri_deriv.argtypes = [c_int, MP, MP, c_double, dptr, dptr]
+        L.simint_eri_worksize.argtypes = [c_int, c_int]
+        L.simint_eri_worksize.restype = c_size_t
+        L.simint_primscreen.argtypes = [SP, SP, dptr, c_int]
+        L.simint_primscreen.restype = c_double
+        L.simint_shellscreen.argtypes = [SP, SP, c_int]
+        L.simint_shellscreen.restype = c_double
+        L.simint_hip_pairs_create.argtypes = [MP, ctypes.POINTER(c_void_p)]
+        L.simint_hip_pairs_free.argtypes = [c_void_p]
+        L.simint_hip_pairs_count.argtypes = [c_void_p]
+        L.simint_hip_compute_eri_prefix.argtypes = [c_void_p, c_void_p, c_int, c_int, iptr, llptr, c_double, c_void_p]
+        L.simint_hip_compute_eri_prefix.restype = ctypes.c_longlong
+        L.simint_hip_compute_eri_diag.argtypes = [c_void_p, c_void_p]
+        L.simint_hip_compute_eri_diag.restype = ctypes.c_longlong
+        L.simint_hip_stream.restype = c_void_p
+        L.simint_hip_last_error.restype = ctypes.c_char_p
+        L.simint_hip_class_kernel.argtypes = [c_int] * 4
+        L.simint_hip_set_class_kernel.argtypes = [c_int] * 5
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    return lib().simint_hip_last_error().decode()
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+class Basis:
+    """A list of simint_shell structures created through the C API.
+
+    shells: iterable of (am, (x, y, z), alphas, coefs).  Shells are
+    normalised with simint_normalize_shells when normalize=True."""
+
+    def __init__(self, shells, normalize=True):
+        shells = list(shells)
+        self.n = len(shells)
+        self.arr = (Shell * max(self.n, 1))()
+        L = lib()
+        for i, (am, xyz, al, co) in enumerate(shells):
+            a = np.ascontiguousarray(al, dtype=np.float64)
+            c = np.ascontiguousarray(co, dtype=np.float64)
+            L.simint_initialize_shell(ctypes.byref(self.arr[i]))
+            L.simint_create_shell(len(a), int(am), float(xyz[0]), float(xyz[1]), float(xyz[2]),
+                                  a.ctypes.data_as(dptr), c.ctypes.data_as(dptr), ctypes.byref(self.arr[i]))
+        if normalize and self.n:
+            L.simint_normalize_shells(self.n, self.arr)
+
+    def am(self, i):
+        return self.arr[i].am
+
+    def subset(self, idx):
+        """ctypes array of copies of the structs (same storage, so the same
+        shell object for compare_shell)."""
+        out = (Shell * max(len(idx), 1))()
+        for k, i in enumerate(idx):
+            out[k] = self.arr[i]
+        return out
+
+    def free(self):
+        L = lib()
+        for i in range(self.n):
+            L.simint_free_shell(ctypes.byref(self.arr[i]))
+        self.n = 0
+
+    def coefs(self, i):
+        s = self.arr[i]
+        return np.ctypeslib.as_array(s.coef, shape=(s.nprim,)).copy()
+
+
+class Pairs:
+    """A simint_multi_shellpair built with simint_create_multi_shellpair."""
+
+    def __init__(self, basis, a_idx, b_idx, screen=SCREEN_NONE):
+        self.mp = MultiShellpair()
+        L = lib()
+        L.simint_initialize_multi_shellpair(ctypes.byref(self.mp))
+        A = basis.subset(a_idx)
+        B = basis.subset(b_idx)
+        L.simint_create_multi_shellpair(len(a_idx), A, len(b_idx), B, ctypes.byref(self.mp), screen)
+        self.n = len(a_idx) * len(b_idx)
+
+    @classmethod
+    def from_pairs(cls, basis, pairs, screen=SCREEN_NONE):
+        """simint_create_multi_shellpair2 on an explicit list of (i, j)."""
+        self = cls.__new__(cls)
+        self.mp = MultiShellpair()
+        L = lib()
+        L.simint_initialize_multi_shellpair(ctypes.byref(self.mp))
+        AB = (Shell * max(2 * len(pairs), 1))()
+        for k, (i, j) in enumerate(pairs):
+            AB[2 * k] = basis.arr[i]
+            AB[2 * k + 1] = basis.arr[j]
+        L.simint_create_multi_shellpair2(len(pairs), AB, ctypes.byref(self.mp), screen)
+        self.n = len(pairs)
+        return self
+
+    def field(self, name):
+        n = self.mp.nprim if name not in ("AB_x", "AB_y", "AB_z", "nprim12") else self.mp.nshell12
+        p = getattr(self.mp, name)
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def free(self):
+        lib().simint_free_multi_shellpair(ctypes.byref(self.mp))
+
+
+def compute_eri(P, Q, screen_tol=0.0, deriv=0):
+    """Drop-in call; returns (rc, integrals as a flat numpy array)."""
+    la, lb, lc, ld = P.mp.am1, P.mp.am2, Q.mp.am1, Q.mp.am2
+    n = P.mp.nshell12_clip * Q.mp.nshell12_clip * ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    n *= 12 if deriv else 1
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    if deriv:
+        rc = lib().simint_compute_eri_deriv(deriv, ctypes.byref(P.mp), ctypes.byref(Q.mp), screen_tol, None,
+                                            out.ctypes.data_as(dptr))
+    else:
+        rc = lib().simint_compute_eri(ctypes.byref(P.mp), ctypes.byref(Q.mp), screen_tol, None,
+                                      out.ctypes.data_as(dptr))
+    return rc, out[:n]
+
+
+class DevicePairs:
+    """simint_hip_pairs: shell pairs resident in HBM."""
+
+    def __init__(self, pairs):
+        h = c_void_p()
+        rc = lib().simint_hip_pairs_create(ctypes.byref(pairs.mp), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError("simint_hip_pairs_create failed (%d): %s" % (rc, last_error()))
+        self.h = h
+        self.am = (pairs.mp.am1, pairs.mp.am2)
+        self.n = pairs.mp.nshell12_clip
+
+    def free(self):
+        if self.h:
+            lib().simint_hip_pairs_free(self.h)
+            self.h = None
+
+
+def compute_prefix(P, Q, b0, nket, out_off, d_out_ptr, screen_tol=0.0):
+    nket = np.ascontiguousarray(nket, dtype=np.int32)
+    out_off = np.ascontiguousarray(out_off, dtype=np.int64)
+    r = lib().simint_hip_compute_eri_prefix(P.h, Q.h, int(b0), len(nket), nket.ctypes.data_as(iptr),
+                                            out_off.ctypes.data_as(llptr), float(screen_tol), c_void_p(d_out_ptr))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_eri_prefix failed (%d): %s" % (r, last_error()))
+    return r
+
+
+def compute_diag(P, d_out_ptr):
+    r = lib().simint_hip_compute_eri_diag(P.h, c_void_p(d_out_ptr))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_eri_diag failed (%d): %s" % (r, last_error()))
+    return r
+
+
+def synchronize():
+    rc = lib().simint_hip_synchronize()
+    if rc != 0:
+        raise RuntimeError("simint_hip_synchronize failed (%d): %s" % (rc, last_error()))

@@ -1,0 +1,204 @@
+"""Parity of the GPU ERI path (through the C ABI) against the CPU oracle.
+
+Bar: per shell quartet, max|gpu - oracle| / max|oracle| <= 1e-14
+("normwise relative"), the FP64 drop-in tolerance; the oracle itself is
+pinned against an independent McMurchie-Davidson brute force in
+test_oracle.py.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+import simint
+import workloads
+from helpers import mol_shells, normalized_arrays, per_quartet_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-14
+
+
+def _oracle_for_dropin(sa, A, B, C, D, screen_method=0, screen_tol=0.0, deriv=0):
+    """Oracle output laid out as simint_compute_eri writes P=(A x B), Q=(C x D)."""
+    quartets = [(a, b, c, d) for a in A for b in B for c in C for d in D]
+    out, offs, sizes = oracle.eri(sa, quartets, screen_method, screen_tol, deriv)
+    return out, offs, sizes
+
+
+def _check(sa, basis, A, B, C, D, screen=0, tol=0.0):
+    P = simint.Pairs(basis, A, B, screen)
+    Q = simint.Pairs(basis, C, D, screen)
+    try:
+        rc, got = simint.compute_eri(P, Q, tol)
+    finally:
+        P.free()
+        Q.free()
+    ref, offs, sizes = _oracle_for_dropin(sa, A, B, C, D, screen, tol)
+    if rc == -1:
+        return np.zeros(len(sizes))
+    assert rc == len(A) * len(B) * len(C) * len(D), simint.last_error()
+    return per_quartet_errors(got, ref, offs, sizes)
+
+
+def test_config1_pppp_single_quartets(gpu):
+    """1000 (pp|pp) quartets, one simint_compute_eri call each."""
+    shells, quartets = workloads.config1()
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    worst = 0.0
+    for (a, b, c, d) in quartets:
+        e = _check(sa, basis, [a], [b], [c], [d])
+        worst = max(worst, e.max())
+    print("config1 worst normwise rel", worst)
+    assert worst <= TOL
+
+
+def test_config1_pppp_batched(gpu):
+    shells, _ = workloads.config1()
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    e = _check(sa, basis, list(range(0, 6)), list(range(0, 6)), list(range(10, 25)), list(range(20, 40)))
+    assert e.max() <= TOL
+
+
+@pytest.mark.parametrize("mol", ["water.6-31gss", "water.sto-3g"])
+def test_all_classes_molecule(gpu, mol):
+    """Every class up to d for a molecule from simint's test fixtures:
+    P = all shells of AM i x all of AM j, Q = all of AM k x all of AM l
+    (the shape skel/test/test_eri.cpp:103-168 uses, with P batched)."""
+    shells = mol_shells(mol)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    ams = sorted(a for a in byam if a <= 2)
+    worst = {}
+    for i, j, k, l in itertools.product(ams, repeat=4):
+        e = _check(sa, basis, byam[i], byam[j], byam[k], byam[l])
+        worst[(i, j, k, l)] = e.max()
+    bad = {k: v for k, v in worst.items() if v > TOL}
+    print("worst over classes", max(worst.values()))
+    assert not bad, bad
+
+
+def test_lane_and_wave_kernels_agree(gpu):
+    """Both kernel families (lane-per-quartet in registers, wave-per-quartet
+    in LDS) against the oracle for every class that has both."""
+    shells = workloads.config2(natom=2)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    L = simint.lib()
+    checked = 0
+    for am in itertools.product(range(3), repeat=4):
+        prev = L.simint_hip_class_kernel(*am)
+        for fam in (1, 2):
+            if L.simint_hip_set_class_kernel(*am, fam) < 0:
+                continue
+            e = _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]])
+            assert e.max() <= TOL, (am, fam, e.max())
+            checked += 1
+        L.simint_hip_set_class_kernel(*am, prev)
+    assert checked >= 81
+
+
+@pytest.mark.parametrize("screen", [simint.SCREEN_FASTSCHWARZ, simint.SCREEN_SCHWARZ])
+@pytest.mark.parametrize("tol", [1e-4, 1e-7])
+def test_primitive_screening(gpu, screen, tol):
+    """Primitive screening (OSTEI_Writer.cpp:504-587, one primitive quartet at
+    a time) matches the oracle's restatement, including the primitive sort the
+    screened pairs get (shell.c:579-581)."""
+    shells = workloads.config2(natom=2, seed=3)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    for am in [(0, 0, 0, 0), (1, 0, 1, 0), (1, 1, 1, 1), (2, 0, 1, 0), (2, 1, 2, 0)]:
+        e = _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]], screen, tol)
+        assert e.max() <= TOL, (am, e.max())
+
+
+def test_empty_and_clipped(gpu):
+    shells, _ = workloads.config1()
+    basis = simint.Basis(shells)
+    P = simint.Pairs(basis, [0, 1], [2, 3])
+    Q = simint.Pairs(basis, [4], [5])
+    P.mp.nshell12_clip = 0
+    rc, _ = simint.compute_eri(P, Q)
+    assert rc == 0
+    P.mp.nshell12_clip = 3       # first three of the four pairs
+    rc, got = simint.compute_eri(P, Q)
+    assert rc == 3
+    sa = normalized_arrays(shells)
+    ref, offs, sizes = oracle.eri(sa, [(0, 2, 4, 5), (0, 3, 4, 5), (1, 2, 4, 5)])
+    assert per_quartet_errors(got, ref, offs, sizes).max() <= TOL
+    P.free()
+    Q.free()
+
+
+def test_extreme_geometry(gpu):
+    """Coincident centres (Boys argument 0), far-apart tight shells (long-range
+    branch x >= 36.5) and very diffuse exponents in one set."""
+    shells = [
+        (0, (0.0, 0.0, 0.0), [5000.0, 0.05], [0.3, 0.7]),
+        (1, (0.0, 0.0, 0.0), [2.0], [1.0]),
+        (2, (0.0, 0.0, 0.0), [0.8], [1.0]),
+        (1, (25.0, -3.0, 11.0), [30.0, 4.0], [0.5, 0.5]),
+        (2, (25.0, -3.0, 11.0), [12.0], [1.0]),
+        (0, (40.0, 40.0, 40.0), [1e4], [1.0]),
+    ]
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    for q in itertools.product(range(len(shells)), repeat=4):
+        e = _check(sa, basis, [q[0]], [q[1]], [q[2]], [q[3]])
+        assert e.max() <= TOL, (q, e.max())
+
+
+def test_device_prefix_and_diag(gpu):
+    """simint_hip_compute_eri_prefix / _diag: device-resident pairs, per-bra
+    ket prefixes, integrals left in HBM."""
+    import torch
+    shells = workloads.config2(natom=3, seed=1)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    rng = np.random.default_rng(0)
+    for (la, lb, lc, ld) in [(0, 0, 0, 0), (1, 1, 0, 0), (2, 1, 1, 0), (2, 2, 2, 2), (1, 0, 2, 1)]:
+        A, B, C, D = byam[la], byam[lb], byam[lc], byam[ld]
+        Pp = simint.Pairs(basis, A, B)
+        Qp = simint.Pairs(basis, C, D)
+        P, Q = simint.DevicePairs(Pp), simint.DevicePairs(Qp)
+        nc = simint.ncart(la) * simint.ncart(lb) * simint.ncart(lc) * simint.ncart(ld)
+        b0 = 1
+        nbra = P.n - 1
+        nket = rng.integers(0, Q.n + 1, size=nbra)
+        off = np.zeros(nbra, dtype=np.int64)
+        off[1:] = np.cumsum(nket[:-1]) * nc
+        out = torch.zeros(int(nket.sum()) * nc + 1, dtype=torch.float64, device="cuda")
+        simint.compute_prefix(P, Q, b0, nket, off, out.data_ptr())
+        simint.synchronize()
+        got = out.cpu().numpy()
+        pairsP = [(a, b) for a in A for b in B]
+        pairsQ = [(c, d) for c in C for d in D]
+        quartets = []
+        for i in range(nbra):
+            for k in range(nket[i]):
+                quartets.append(pairsP[b0 + i] + pairsQ[k])
+        ref, offs, sizes = oracle.eri(sa, quartets)
+        assert per_quartet_errors(got, ref, offs, sizes).max() <= TOL
+        # diagonal (bb|bb)
+        if (la, lb) == (lc, ld):
+            outd = torch.zeros(P.n * nc, dtype=torch.float64, device="cuda")
+            simint.compute_diag(P, outd.data_ptr())
+            simint.synchronize()
+            refd, offs, sizes = oracle.eri(sa, [p + p for p in pairsP])
+            assert per_quartet_errors(outd.cpu().numpy(), refd, offs, sizes).max() <= TOL
+        P.free(); Q.free(); Pp.free(); Qp.free()
