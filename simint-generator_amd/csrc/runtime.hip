@@ -96,10 +96,6 @@ struct DevBuf {
         if (e == hipSuccess) cap = want;
         return e;
     }
-    ~DevBuf()
-    {
-        if (p) hipFree(p);
-    }
 };
 
 struct PinBuf {
@@ -116,27 +112,22 @@ struct PinBuf {
         if (e == hipSuccess) cap = want;
         return e;
     }
-    ~PinBuf()
-    {
-        if (p) hipHostFree(p);
-    }
 };
 
+// Per-thread stream and buffers.  They are never released: freeing HIP
+// resources from thread-exit or static destructors can race the runtime's
+// own teardown, and the process exit returns the memory anyway.
 struct ThreadCtx {
     hipStream_t s = nullptr;
     DevBuf in, out, items;
     PinBuf stage;
     ThreadCtx() { hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
-    ~ThreadCtx()
-    {
-        if (s) hipStreamDestroy(s);
-    }
 };
 
 ThreadCtx &ctx()
 {
-    thread_local std::unique_ptr<ThreadCtx> c;
-    if (!c) c.reset(new ThreadCtx());
+    thread_local ThreadCtx *c = nullptr;
+    if (!c) c = new ThreadCtx();
     return *c;
 }
 
@@ -436,6 +427,7 @@ size_t simint_eri_workmem(int derorder, int maxam) { return simint_ostei_workmem
 // --------------------------------------------------------- device-resident
 struct simint_hip_pairs {
     int am1, am2, n;
+    bool has_screen;
     double screen_max;
     void *dev = nullptr;
     PairsLinear lin;
@@ -465,6 +457,7 @@ int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **
     p->am2 = P->am2;
     p->n = n;
     p->screen_max = P->screen_max;
+    p->has_screen = P->screen != nullptr;
     p->dev = d;
     p->lin = linear_ptrs(static_cast<char *>(d), lo);
     p->til = tiled_ptrs(static_cast<char *>(d), to);
@@ -560,7 +553,7 @@ long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_
     a.nitems = nitems;
     a.out = d_out;
     a.boys_grid = g_dev.boys;
-    a.tol2 = screen_tol > 0.0 ? screen_tol * screen_tol : 0.0;
+    a.tol2 = (screen_tol > 0.0 && P->has_screen && Q->has_screen) ? screen_tol * screen_tol : 0.0;
     a.ket_screen_max = Q->screen_max;
     a.stride = ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
     if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
