@@ -28,6 +28,10 @@
 
 typedef long double ld;
 
+/* oracle.c: simint's split Boys scheme (reference values, double) */
+void oracle_boys(double x, int n, double *F);
+static int g_refboys = 0;   /* 1: use the reference scheme's F_m (test mode) */
+
 static ld md_boys(int n, ld x)
 {
     if (x > 50.0L) {
@@ -128,9 +132,11 @@ static void md_quartet(const md_shell *S[4], ld *out)
                     }
                     /* R^n_{tuv}, n from N down to 0 */
                     const ld T = alpha * RPQ2;
+                    double Fref[64];
+                    if (g_refboys) oracle_boys((double)T, N, Fref);
                     for (int n = N; n >= 0; n--) {
                         const int lim = N - n;
-                        ld f = md_boys(n, T);
+                        ld f = g_refboys ? (ld)Fref[n] : md_boys(n, T);
                         ld m2a = 1.0L;
                         for (int z = 0; z < n; z++) m2a *= -2.0L * alpha;
                         for (int t = 0; t <= lim; t++)
@@ -203,6 +209,11 @@ static void load(int nsh, const int *am, const int *nprim, const int *prim_off, 
         s[i].coef = coef + prim_off[i];
     }
 }
+
+/* Select the Boys values md_eri uses: 0 = exact (series / asymptotic in long
+ * double), 1 = simint's split scheme (oracle_boys), which isolates the
+ * recurrence algebra from the Boys-function approximation. */
+void md_set_reference_boys(int on) { g_refboys = on; }
 
 /* contracted integrals for a list of shell quartets (same layout and offsets
  * as oracle_eri_quartets with deriv = 0) */

@@ -39,6 +39,7 @@ def lib():
         L.oracle_boys_grid.argtypes = [dp]
         L.oracle_boys_longfac.argtypes = [dp]
         L.oracle_class_counts.argtypes = [ctypes.c_int] * 4 + [llp]
+        L.md_set_reference_boys.argtypes = [ctypes.c_int]
         L.md_boys_value.argtypes = [ctypes.c_int, ctypes.c_double]
         L.md_boys_value.restype = ctypes.c_double
         _LIB = L
@@ -104,11 +105,16 @@ def eri(sa, quartets, screen_method=0, screen_tol=0.0, deriv=0):
     return out, off, n
 
 
-def md_eri(sa, quartets):
+def md_eri(sa, quartets, reference_boys=False):
+    """Brute-force McMurchie-Davidson values in long double.  With
+    reference_boys=True the Boys values are simint's split scheme (so only
+    the recurrence algebra is checked)."""
     q, n, off = quartet_sizes(sa, quartets, 0)
     out = np.zeros(int(n.sum()), dtype=np.float64)
+    lib().md_set_reference_boys(1 if reference_boys else 0)
     rc = lib().md_eri_quartets(*sa.args(), len(q), _p(q, ctypes.c_int), _p(off, ctypes.c_longlong),
                                _p(out, ctypes.c_double))
+    lib().md_set_reference_boys(0)
     if rc != 0:
         raise ValueError("md_eri: angular momentum too high")
     return out, off, n

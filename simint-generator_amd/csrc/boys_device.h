@@ -7,7 +7,12 @@
 //              F_m = (2x F_{m+1} + e^-x) / (2m+1) for the lower orders
 //              (boys.h:69-80);
 //   x >= 36.5: F_m = longfac[m] x^-(m+1/2), longfac[m] = sqrt(pi)(2m-1)!!/2^(m+1)
-//              (boys_long.h:14-27).
+//              (boys_long.h:14-27) for every order when L < 4; for L >= 4
+//              only F_L, then the same downward recursion (boys.h:49-80).
+//              The asymptotic form drops Gamma(m+1/2, x), so near the cutoff
+//              the reference's F_m carry a relative error that grows with m
+//              (about 4e-12 for m = 4 at x = 36.5); the device follows the
+//              reference's branch structure so both give the same values.
 // The reference evaluates every order by Taylor when L < 4; here the
 // recursion is used for every L >= 1, which trades 8(L) table loads for one
 // exp(), and agrees to a few ulp.  The table is built by the package's own
@@ -41,7 +46,8 @@ __device__ __forceinline__ void boys_eval(const double *__restrict__ grid, doubl
             for (int n = L - 1; n >= 0; n--)
                 F[n] = (1.0 / (2 * n + 1)) * fma(x2, F[n + 1], ex);
         }
-    } else {
+    } else if (L < 4) {
+        // every order from the long-range form (boys.h:18-25, boys_long.h:14-27)
         const double x1 = 1.0 / x;
         double x2 = sqrt(x1);
 #pragma unroll
@@ -49,5 +55,15 @@ __device__ __forceinline__ void boys_eval(const double *__restrict__ grid, doubl
             F[n] = longfac[n] * x2;
             x2 *= x1;
         }
+    } else {
+        // top order from the long-range form, then down (boys.h:49-80,
+        // boys_long.h:29-35): this keeps the reference's values, whose
+        // e^-x tail enters through the recursion
+        F[L] = longfac[L] * sqrt(pow(x, -(double)(2 * L + 1)));
+        const double x2 = 2.0 * x;
+        const double ex = exp(-x);
+#pragma unroll
+        for (int n = L - 1; n >= 0; n--)
+            F[n] = (1.0 / (2 * n + 1)) * fma(x2, F[n + 1], ex);
     }
 }
