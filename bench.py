@@ -1,0 +1,199 @@
+"""Throughput benchmark of the MI355X ERI engine (one JSON line on stdout).
+
+Workload: the 64-atom, all-classes-up-to-d system (jittered 3-bohr cubic
+grid; per atom s(6) s(3) p(3) core and s(1) p(1) d(1) valence shells), every
+unique shell quartet passing a Schwarz screen of 1e-12, FP64.  One step is
+one pass over all of those quartets through the device-resident batched path
+(simint_hip_compute_eri_prefix), integrals written to HBM (a reused output
+buffer: the full set is ~0.85 TB of integrals).  Inputs are resident before
+the timed region.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--natom 64] [--no-cpu]
+
+Besides the metric the line carries:
+  roofline      for the kernel with the largest share of device time:
+                algorithmic FP64 FLOPs per launch (FMA = 2; per quartet
+                K (f_setup + f_vrr + f_boys + bcont) + f_hrr, the reference
+                generator's operation counts) over its mean launch time,
+                measured with HIP events on the library stream in the timed
+                region, against the 78.6 TFLOP/s FP64 peak;
+  cpu_baseline  the CPU oracle (a C restatement of the reference algorithm,
+                one core) on a random sample of the same quartets.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "simint-generator_amd")
+
+METRIC = "shell quartets/s at ≤1e-10 abs err, mixed vs FP64, 1 GPU; p50/p99 latency"
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def cpu_baseline(ss, seconds=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sa = oracle.ShellArrays(ss.shells)
+    oracle.normalize(sa)
+    done = 0
+    t_used = 0.0
+    chunk = 200
+    seed = 0
+    while t_used < seconds:
+        qs = ss.sample(chunk, seed=seed)
+        seed += 1
+        t0 = time.perf_counter()
+        oracle.eri(sa, qs)
+        t_used += time.perf_counter() - t0
+        done += len(qs)
+        if t_used < seconds / 4:
+            chunk = min(chunk * 2, 20000)
+    return {"value": done / t_used, "unit": "shell quartets/s", "cores": 1, "kind": "port",
+            "sample": "%d quartets drawn uniformly from the screened 64-atom set, %.1f s of oracle time"
+                      % (done, t_used)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--natom", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-class", default=None,
+                    help="profiling aid: run only this class's launches (e.g. d_p_d_p)")
+    ap.add_argument("--copy-gib", type=float, default=None,
+                    help="profiling aid: run one device copy of this many GiB and exit "
+                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(shell quartets are independent; sharding them over GPUs is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if args.copy_gib:
+        n = int(args.copy_gib * (1 << 30)) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 8 * n}))
+        return
+    sys.path.insert(0, PKG)
+    import quartets
+    import simint
+    import workloads
+
+    shells = workloads.config2(natom=args.natom, seed=0)
+    t_setup = time.perf_counter()
+    ss = quartets.ScreenedSet(shells, tol=1e-12)
+    t_setup = time.perf_counter() - t_setup
+    if args.only_class:
+        ss.launches = [ln for ln in ss.launches if quartets.simint_class_name(ln.cls) == args.only_class]
+        ss.total_quartets = sum(ln.nq for ln in ss.launches)
+        ss.total_flops = sum(ln.flops for ln in ss.launches)
+        ss.total_bytes = sum(8.0 * ln.nout for ln in ss.launches)
+    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
+
+    for _ in range(args.warmup):
+        ss.run_pass(out.data_ptr())
+    simint.synchronize()
+    torch.cuda.synchronize()
+
+    events = []
+
+    def rec(ln, start):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        events.append((ln, start, e))
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ss.run_pass(out.data_ptr(), events=rec)
+    simint.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = ss.total_quartets * args.steps / elapsed
+
+    # per-class device time from the events
+    per = {}
+    for k in range(0, len(events), 2):
+        ln, _, e0 = events[k]
+        _, _, e1 = events[k + 1]
+        ms = e0.elapsed_time(e1)
+        c = per.setdefault(ln.cls, {"ms": 0.0, "flops": 0.0, "bytes": 0.0, "launches": 0, "quartets": 0})
+        c["ms"] += ms
+        c["flops"] += ln.flops
+        c["bytes"] += 8.0 * ln.nout
+        c["launches"] += 1
+        c["quartets"] += ln.nq
+    dom = max(per, key=lambda k: per[k]["ms"])
+    d = per[dom]
+    mean_s = d["ms"] / 1000.0 / d["launches"]
+    flops_per_launch = d["flops"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    t_flop = flops_per_launch / PEAK_FP64
+    t_byte = bytes_per_launch / PEAK_HBM
+    if t_flop >= t_byte:
+        achieved = flops_per_launch / mean_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / (PEAK_FP64 / 1e12), "traffic": None}
+    else:
+        achieved = bytes_per_launch / mean_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": achieved / (PEAK_HBM / 1e9), "traffic": None}
+    info = ss.info[quartets.simint_class_name(dom)]
+    roof.update({"kernel": "eri_%s_%s" % (info["kernel"], quartets.simint_class_name(dom)),
+                 "compute": "FP64 VALU (no MFMA); peak is the FP64 vector rate",
+                 "launches": d["launches"], "mean_launch_ms": 1000.0 * mean_s,
+                 "flops_per_launch": flops_per_launch, "share_of_device_time":
+                     d["ms"] / sum(v["ms"] for v in per.values())})
+    whole = {"tflops": ss.total_flops * args.steps / elapsed / 1e12,
+             "frac_fp64_peak": ss.total_flops * args.steps / elapsed / PEAK_FP64,
+             "output_GBps": ss.total_bytes * args.steps / elapsed / 1e9}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "shell quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%d-atom jittered 3-bohr grid, s6 s3 p3 s1 p1 d1 per atom, all unique "
+                               "quartets with Schwarz bound >= 1e-12, plain FP64 (l <= 2)" % args.natom,
+                   "natom": args.natom, "nshell": len(shells), "quartets_per_step": ss.total_quartets,
+                   "launches_per_step": len(ss.launches), "path": "device-resident batched",
+                   "setup_s": t_setup},
+        "roofline": roof,
+        "whole_job": whole,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(ss)
+    top = sorted(per.items(), key=lambda kv: -kv[1]["ms"])[:8]
+    line["top_classes"] = [{"class": quartets.simint_class_name(k), "ms_per_step": v["ms"] / args.steps,
+                            "tflops": v["flops"] / (v["ms"] / 1000.0) / 1e12,
+                            "quartets_per_step": v["quartets"] // args.steps,
+                            "kernel": ss.info[quartets.simint_class_name(k)]["kernel"]} for k, v in top]
+    print(json.dumps(line))
+    ss.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""Per-class kernel-family autotuner (runs on the GPU).
+
+For every class that has both kernel families built, times the
+lane-per-quartet (registers) and wave-per-quartet (LDS-staged, batched)
+kernels on the screened quartets of a synthetic system built with the
+64-atom benchmark recipe at a smaller size, and writes the faster family per
+class to tuning.json (read by the build to set each class's default):
+
+    python simint-generator_amd/autotune.py [--natom 16] [--reps 3] [--out PATH]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--natom", type=int, default=16)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
+    a = ap.parse_args()
+    import torch
+    import quartets
+    import simint
+    import workloads
+    L = simint.lib()
+    ss = quartets.ScreenedSet(workloads.config2(natom=a.natom, seed=0), tol=1e-12)
+    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(L.simint_hip_stream())
+    by = {}
+    for ln in ss.launches:
+        by.setdefault(ln.cls, []).append(ln)
+    family = {}
+    timings = {}
+    for cls in sorted(by):
+        name = quartets.simint_class_name(cls)
+        prev = L.simint_hip_class_kernel(*cls)
+        res = {}
+        for fam, fname in ((1, "lpq"), (2, "wpq")):
+            if L.simint_hip_set_class_kernel(*cls, fam) < 0:
+                continue
+            best = None
+            for rep in range(a.reps + 1):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for ln in by[cls]:
+                    simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out.data_ptr(), 0.0)
+                e1.record(stream)
+                simint.synchronize()
+                ms = e0.elapsed_time(e1)
+                if rep > 0:
+                    best = ms if best is None else min(best, ms)
+            res[fname] = best
+        L.simint_hip_set_class_kernel(*cls, prev)
+        if res:
+            family[name] = min(res, key=res.get)
+            timings[name] = res
+            print("%-8s %s" % (name, "  ".join("%s %.3f ms" % kv for kv in sorted(res.items()))), flush=True)
+    doc = {"workload": "config-2 recipe, %d atoms, Schwarz 1e-12, quartets of each class" % a.natom,
+           "family": family, "timings_ms": timings,
+           "generated": time.strftime("%Y-%m-%d %H:%M:%S")}
+    with open(a.out, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    print("wrote", a.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -44,17 +44,35 @@ def levels(g):
 
 
 def plan(am):
+    """Slot layout and node descriptors of the LDS-staged VRR.
+
+    Slots are ordered by dependency level; within a level each component
+    quartet owns a run of consecutive slots, one per m in its (contiguous)
+    m range, so the (m, m+1) source pairs of a step sit in adjacent slots and
+    one index addresses both.  Bases (ss|ss)^m occupy slots 0..L.
+    Descriptor (two 64-bit words):
+      w0 = s0 | s2 << 16 | s4 << 32 | s6 << 48
+      w1 = s7 | flags << 16 | f0 << 24 | f1 << 32 | f2 << 40 | f3 << 48
+      flags: bit0 ket step, bits1-2 direction, bit3..6 s2/s4/s6/s7 present."""
     g = graph(*am)
     lev = levels(g)
-    nodes = sorted(g.node_list, key=lambda n: (lev[(n.q, n.m)], n.m))
-    # bases first, in m order, so slot m holds F_m
-    bases = [n for n in nodes if n.step is None]
-    rest = [n for n in nodes if n.step is not None]
-    order = sorted(bases, key=lambda n: n.m) + rest
+    comps = {}
+    for n in g.node_list:
+        comps.setdefault(n.q, []).append(n)
+    bases = sorted(comps.pop(((0, 0, 0),) * 4), key=lambda n: n.m)
+    order = list(bases)
+    by_level = {}
+    for q, ns in comps.items():
+        by_level.setdefault(lev[(q, ns[0].m)], []).append(q)
+    for lv in sorted(by_level):
+        for q in sorted(by_level[lv]):
+            ms = sorted(comps[q], key=lambda n: n.m)
+            assert [n.m for n in ms] == list(range(ms[0].m, ms[-1].m + 1))
+            order.extend(ms)
     slot = {(n.q, n.m): i for i, n in enumerate(order)}
-    nlev = max(lev.values()) + 1 if lev else 1
-    starts = []
-    cur = -1
+    assert len(order) < 65536
+    starts = [0]
+    cur = 0
     for i, n in enumerate(order):
         lv = lev[(n.q, n.m)]
         while cur < lv:
@@ -64,19 +82,45 @@ def plan(am):
     desc = []
     for n in order:
         if n.step is None:
-            desc.append((0, 0, 0))
+            desc.append((0, 0))
             continue
-        s = [slot[x] if x is not None else 0 for x in n.src]
-        w0 = s[0] | (s[1] << 16) | (s[2] << 32) | (s[3] << 48)
-        w1 = s[4] | (s[5] << 16) | (s[6] << 32) | (s[7] << 48)
-        isket = 1 if n.step in (K, L) else 0
+        sl = [slot[x] if x is not None else 0 for x in n.src]
+        for a, b in ((0, 1), (2, 3), (4, 5)):
+            if n.src[a] is not None:
+                assert sl[b] == sl[a] + 1
         has = [n.src[2] is not None, n.src[4] is not None, n.src[6] is not None, n.src[7] is not None]
+        flags = (1 if n.step in (K, L) else 0) | (n.dir << 1) | (has[0] << 3) | (has[1] << 4) \
+            | (has[2] << 5) | (has[3] << 6)
         f = n.fac
-        w2 = (isket | (n.dir << 1) | (has[0] << 3) | (has[1] << 4) | (has[2] << 5) | (has[3] << 6)
-              | (f[0] << 8) | (f[1] << 16) | (f[2] << 24) | (f[3] << 32))
-        desc.append((w0, w1, w2))
+        assert max(f) < 256
+        w0 = sl[0] | (sl[2] << 16) | (sl[4] << 32) | (sl[6] << 48)
+        w1 = sl[7] | (flags << 16) | (f[0] << 24) | (f[1] << 32) | (f[2] << 40) | (f[3] << 48)
+        desc.append((w0, w1))
     tops = [slot[k] for k in g.top_keys]
     return g, order, starts, desc, tops
+
+
+LDS_BUDGET = 56 * 1024
+THREADS = 256
+
+
+def batch_size(am):
+    """Quartets per batch: the largest power of two <= 16 whose LDS image
+    (VRR/HRR work area per quartet plus the primitive-constant batch) fits the
+    per-workgroup budget (so 2-4 workgroups share a CU)."""
+    g, order, starts, desc, tops = plan(am)
+    la, lb, lc, ld = am
+    ws = max(len(order), len(tops) + ncart(la) * ncart(lb) * len(g.ket_tops()))
+    npc = NPC_FIXED + g.Ltot + 1
+    B = 16
+    while B > 1:
+        if 8 * (ws * B + PB_SETUP * npc * B) <= LDS_BUDGET:
+            break
+        B //= 2
+    return B, ws, npc
+
+
+PB_SETUP = 4   # primitive quartets per setup batch
 
 
 def emit_wpq(am):
@@ -86,37 +130,42 @@ def emit_wpq(am):
     Ltot = g.Ltot
     nnode = len(order)
     ntop = len(tops)
-    nacc = (ntop + 63) // 64
-    npc = NPC_FIXED + Ltot + 1
+    B, ws, npc = batch_size(am)
+    LPB = THREADS // B
+    nacc = (ntop + LPB - 1) // LPB
     bra_tops = g.bra_tops()
     ket_tops = g.ket_tops()
     ne, nf = len(bra_tops), len(ket_tops)
     nab = ncart(la) * ncart(lb)
     ncd = ncart(lc) * ncart(ld)
-    lds_work = max(nnode, ntop + nab * nf)
     w = _Writer()
     w("// Generated by simint-generator_amd/emitter/hip_wpq.py -- class (%s)" % name.replace("_", ""))
     w('#include "../device_types.h"')
     w('#include "../boys_device.h"')
     w("")
     w("namespace {")
-    w("constexpr int NNODE = %d, NTOP = %d, NACC = %d, NPC = %d, NE = %d, NF = %d, NAB = %d, NCD = %d;"
-      % (nnode, ntop, nacc, npc, ne, nf, nab, ncd))
+    w("constexpr int NNODE = %d, NTOP = %d, NPC = %d, NE = %d, NF = %d, NAB = %d, NCD = %d;"
+      % (nnode, ntop, npc, ne, nf, nab, ncd))
+    w("constexpr int B = %d, LPB = %d, NACC = %d, PB = %d, WS = %d;" % (B, LPB, nacc, PB_SETUP, ws))
     w("constexpr int NLEV = %d;" % (len(starts) - 1))
     w("__constant__ int lev_start[%d] = {%s};" % (len(starts), ", ".join(map(str, starts))))
-    w("__constant__ unsigned long long vrr_desc[%d][3] = {" % nnode)
+    w("__constant__ unsigned long long vrr_desc[%d][2] = {" % nnode)
     for d in desc:
-        w("    {0x%xull, 0x%xull, 0x%xull}," % d)
+        w("    {0x%xull, 0x%xull}," % d)
     w("};")
     w("__constant__ unsigned short top_slot[%d] = {%s};" % (ntop, ", ".join(map(str, tops))))
     w("}  // namespace")
     w("")
-    w("__global__ void __launch_bounds__(64) eri_wpq_%s(const EriArgs A)" % name)
+    w("// %d threads cooperate on B = %d quartets at a time; thread t serves quartet" % (THREADS, B))
+    w("// slot t %% B and is rank t / B of that quartet's LPB = %d threads." % LPB)
+    w("__global__ void __launch_bounds__(%d) eri_wpq_%s(const EriArgs A)" % (THREADS, name))
     w("{")
-    w("    __shared__ double work[%d];" % lds_work)
-    w("    __shared__ double pc[64 * NPC];")
+    w("    __shared__ double work[WS * B];")
+    w("    __shared__ double pc[PB * NPC * B];")
     w("    const WorkItem wi = A.items[blockIdx.x];")
-    w("    const int lane = threadIdx.x;")
+    w("    const int tid = threadIdx.x;")
+    w("    const int b = tid % B;")
+    w("    const int r0 = tid / B;")
     w("    const double *__restrict__ grid = A.boys_grid;")
     w("    const int ib0 = A.bra.poff[wi.bra];")
     w("    const int npb = A.bra.np[wi.bra];")
@@ -128,16 +177,18 @@ def emit_wpq(am):
         w("    const double hAB[3] = {A.bra.abx[wi.bra], A.bra.aby[wi.bra], A.bra.abz[wi.bra]};")
     bra_pa = "pa" if g.bra_j else "pb"
     ket_pa = "pa" if g.ket_l else "pb"
-    w("    for (int ql = wi.lane_lo; ql < wi.lane_hi; ++ql) {")
+    w("    for (int q0 = wi.lane_lo; q0 < wi.lane_hi; q0 += B) {")
+    w("        const int nb = (wi.lane_hi - q0) < B ? (wi.lane_hi - q0) : B;")
+    w("        const int ql = (q0 + b) < SIMINT_TILE ? (q0 + b) : (SIMINT_TILE - 1);")
     w("        const long long tl = (long long)wi.tile * SIMINT_TILE + ql;")
     w("        double acc[NACC];")
     w("#pragma unroll")
     w("        for (int s = 0; s < NACC; ++s) acc[s] = 0.0;")
-    w("        for (int p0 = 0; p0 < K; p0 += 64) {")
-    w("            const int np_batch = (K - p0) < 64 ? (K - p0) : 64;")
+    w("        for (int p0 = 0; p0 < K; p0 += PB) {")
+    w("            const int np_batch = (K - p0) < PB ? (K - p0) : PB;")
     w("            __syncthreads();")
-    w("            if (lane < np_batch) {")
-    w("                const int p = p0 + lane;")
+    w("            if (r0 < np_batch) {")
+    w("                const int p = p0 + r0;")
     w("                const int i = ib0 + p / nq;")
     w("                const long long kj = kt + ql + (long long)SIMINT_TILE * (p % nq);")
     w("                const double P_alpha = A.bra.alpha[i], P_inv = A.bra.inv_alpha[i];")
@@ -159,77 +210,86 @@ def emit_wpq(am):
     w("                double F[%d];" % (Ltot + 1))
     w("                boys_eval<%d>(grid, R2 * alpha, F);" % Ltot)
     w("                const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
-    w("                double *c = pc + lane * NPC;")
-    w("                c[0] = A.bra.%sx[i]; c[1] = A.bra.%sy[i]; c[2] = A.bra.%sz[i];" % (bra_pa, bra_pa, bra_pa))
-    w("                c[3] = A.ket.%sx[kj]; c[4] = A.ket.%sy[kj]; c[5] = A.ket.%sz[kj];" % (ket_pa, ket_pa, ket_pa))
+    w("                double *c = pc + r0 * NPC * B + b;")
+    w("                c[0 * B] = A.bra.%sx[i]; c[1 * B] = A.bra.%sy[i]; c[2 * B] = A.bra.%sz[i];" % (bra_pa, bra_pa, bra_pa))
+    w("                c[3 * B] = A.ket.%sx[kj]; c[4 * B] = A.ket.%sy[kj]; c[5 * B] = A.ket.%sz[kj];" % (ket_pa, ket_pa, ket_pa))
     w("                const double a_over_p = (-alpha) * P_inv;")
     w("                const double a_over_q = alpha * Q_inv;")
-    w("                c[6] = a_over_p * PQx; c[7] = a_over_p * PQy; c[8] = a_over_p * PQz;")
-    w("                c[9] = a_over_q * PQx; c[10] = a_over_q * PQy; c[11] = a_over_q * PQz;")
-    w("                c[12] = a_over_p; c[13] = -a_over_q;")
-    w("                c[14] = 0.5 * P_inv; c[15] = 0.5 * Q_inv; c[16] = 0.5 * one_over_PQalpha_sum;")
+    w("                c[6 * B] = a_over_p * PQx; c[7 * B] = a_over_p * PQy; c[8 * B] = a_over_p * PQz;")
+    w("                c[9 * B] = a_over_q * PQx; c[10 * B] = a_over_q * PQy; c[11 * B] = a_over_q * PQz;")
+    w("                c[12 * B] = a_over_p; c[13 * B] = -a_over_q;")
+    w("                c[14 * B] = 0.5 * P_inv; c[15 * B] = 0.5 * Q_inv; c[16 * B] = 0.5 * one_over_PQalpha_sum;")
     w("#pragma unroll")
-    w("                for (int m = 0; m <= %d; ++m) c[%d + m] = F[m] * prefac;" % (Ltot, NPC_FIXED))
+    w("                for (int m = 0; m <= %d; ++m) c[(%d + m) * B] = F[m] * prefac;" % (Ltot, NPC_FIXED))
     w("            }")
     w("            __syncthreads();")
     w("            for (int pp = 0; pp < np_batch; ++pp) {")
-    w("                const double *c = pc + pp * NPC;")
-    w("                const double k6[6] = {c[0], c[1], c[2], c[3], c[4], c[5]};")
-    w("                const double a6[6] = {c[6], c[7], c[8], c[9], c[10], c[11]};")
-    w("                const double aover[2] = {c[12], c[13]};")
-    w("                const double o2[2] = {c[14], c[15]};")
-    w("                const double o2pq = c[16];")
-    w("                if (lane <= %d) work[lane] = c[%d + lane];" % (Ltot, NPC_FIXED))
+    w("                const double *c = pc + pp * NPC * B + b;")
+    w("                const double k6[6] = {c[0 * B], c[1 * B], c[2 * B], c[3 * B], c[4 * B], c[5 * B]};")
+    w("                const double a6[6] = {c[6 * B], c[7 * B], c[8 * B], c[9 * B], c[10 * B], c[11 * B]};")
+    w("                const double aover[2] = {c[12 * B], c[13 * B]};")
+    w("                const double o2[2] = {c[14 * B], c[15 * B]};")
+    w("                const double o2pq = c[16 * B];")
+    w("                for (int m = r0; m <= %d; m += LPB) work[m * B + b] = c[(%d + m) * B];" % (Ltot, NPC_FIXED))
     w("                __syncthreads();")
-    w("                for (int lv = 1; lv < NLEV; ++lv) {")
-    w("                    const int e = lev_start[lv + 1];")
-    w("                    for (int k = lev_start[lv] + lane; k < e; k += 64) {")
-    w("                        const unsigned long long w0 = vrr_desc[k][0], w1 = vrr_desc[k][1], w2 = vrr_desc[k][2];")
-    w("                        const int isket = (int)(w2 & 1), d = (int)((w2 >> 1) & 3);")
-    w("                        const int sel = isket * 3 + d;")
-    w("                        double t = k6[sel] * work[w0 & 0xffff];")
-    w("                        t = fma(a6[sel], work[(w0 >> 16) & 0xffff], t);")
-    w("                        const double ao = aover[isket], oo = o2[isket];")
-    w("                        if (w2 & 8) t = fma((double)((w2 >> 8) & 0xff) * oo, fma(ao, work[(w0 >> 48) & 0xffff], work[(w0 >> 32) & 0xffff]), t);")
-    w("                        if (w2 & 16) t = fma((double)((w2 >> 16) & 0xff) * oo, fma(ao, work[(w1 >> 16) & 0xffff], work[w1 & 0xffff]), t);")
-    w("                        if (w2 & 32) t = fma((double)((w2 >> 24) & 0xff) * o2pq, work[(w1 >> 32) & 0xffff], t);")
-    w("                        if (w2 & 64) t = fma((double)((w2 >> 32) & 0xff) * o2pq, work[(w1 >> 48) & 0xffff], t);")
-    w("                        work[k] = t;")
-    w("                    }")
-    w("                    __syncthreads();")
-    w("                }")
+    for lv in range(1, len(starts) - 1):
+        S, E = starts[lv], starts[lv + 1]
+        w("                // level %d: nodes [%d, %d)" % (lv, S, E))
+        w("#pragma unroll 1")
+        w("                for (int k = %d + r0; k < %d; k += LPB) {" % (S, E))
+        w("                    {")
+        w("                        const unsigned long long w0 = vrr_desc[k][0], w1 = vrr_desc[k][1];")
+        w("                        const int fl = (int)((w1 >> 16) & 0xff);")
+        w("                        const int isket = fl & 1, sel = isket * 3 + ((fl >> 1) & 3);")
+        w("                        const double *s0 = work + (w0 & 0xffff) * B + b;")
+        w("                        double t = k6[sel] * s0[0];")
+        w("                        t = fma(a6[sel], s0[B], t);")
+        w("                        const double ao = aover[isket], oo = o2[isket];")
+        w("                        if (fl & 8) { const double *s2 = work + ((w0 >> 16) & 0xffff) * B + b;")
+        w("                            t = fma((double)((w1 >> 24) & 0xff) * oo, fma(ao, s2[B], s2[0]), t); }")
+        w("                        if (fl & 16) { const double *s4 = work + ((w0 >> 32) & 0xffff) * B + b;")
+        w("                            t = fma((double)((w1 >> 32) & 0xff) * oo, fma(ao, s4[B], s4[0]), t); }")
+        w("                        if (fl & 32) t = fma((double)((w1 >> 40) & 0xff) * o2pq, work[(w0 >> 48) * B + b], t);")
+        w("                        if (fl & 64) t = fma((double)((w1 >> 48) & 0xff) * o2pq, work[(w1 & 0xffff) * B + b], t);")
+        w("                        work[k * B + b] = t;")
+        w("                    }")
+        w("                }")
+        w("                __syncthreads();")
     w("#pragma unroll")
     w("                for (int s = 0; s < NACC; ++s) {")
-    w("                    const int t = lane + 64 * s;")
-    w("                    if (t < NTOP) acc[s] += work[top_slot[t]];")
+    w("                    const int t = r0 + LPB * s;")
+    w("                    if (t < NTOP) acc[s] += work[top_slot[t] * B + b];")
     w("                }")
     w("                __syncthreads();")
     w("            }")
     w("        }")
-    # contracted tops -> LDS, HRR
     w("        double *tops = work;")
-    w("        double *abf = work + NTOP;")
+    w("        double *abf = work + NTOP * B;")
     w("        __syncthreads();")
     w("#pragma unroll")
     w("        for (int s = 0; s < NACC; ++s) {")
-    w("            const int t = lane + 64 * s;")
-    w("            if (t < NTOP) tops[t] = acc[s];")
+    w("            const int t = r0 + LPB * s;")
+    w("            if (t < NTOP) tops[t * B + b] = acc[s];")
     w("        }")
     w("        __syncthreads();")
     _emit_hrr_coop(g, w, am)
+    w("        __syncthreads();")
     w("    }")
     w("}")
     w("")
     w('extern "C" hipError_t simint_launch_wpq_%s(const EriArgs *a, hipStream_t s)' % name)
     w("{")
     w("    if (a->nitems <= 0) return hipSuccess;")
-    w("    hipLaunchKernelGGL(eri_wpq_%s, dim3((unsigned)a->nitems), dim3(64), 0, s, *a);" % name)
+    w("    hipLaunchKernelGGL(eri_wpq_%s, dim3((unsigned)a->nitems), dim3(%d), 0, s, *a);" % (name, THREADS))
     w("    return hipGetLastError();")
     w("}")
     return w.text()
 
 
 def _emit_hrr_coop(g, w, am):
+    """HRR for the B quartets of a batch: bra HRR with lanes over (ket-top
+    component f, quartet), then ket HRR with lanes over (final bra
+    component, quartet).  LDS layout [value][quartet slot]."""
     la, lb, lc, ld = am
     bra_ops, bra_map, ket_ops, ket_map = g.hrr_program()
     bra_tops = g.bra_tops()
@@ -240,15 +300,14 @@ def _emit_hrr_coop(g, w, am):
     bra_final = [(a, b) for a in carts(la) for b in carts(lb)]
     abidx = {ab: i for i, ab in enumerate(bra_final)}
     ind = "        "
-    # ---- bra HRR: one lane per ket-top component f
-    w(ind + "for (int f = lane; f < NF; f += 64) {")
-    w(ind + "    const double *T = tops + f;")
+    w(ind + "for (int f = r0; f < NF; f += LPB) {")
+    w(ind + "    const double *T = tops + f * B + b;")
     names = {}
     cnt = [0]
 
     def bval(src):
         if src[0] == "top":
-            return "T[%d]" % (etop[src[1]] * nf)
+            return "T[%d * B]" % (etop[src[1]] * nf)
         return names[(src[1], src[2])]
 
     for (dst, s0, s1, sign, d) in bra_ops:
@@ -258,22 +317,22 @@ def _emit_hrr_coop(g, w, am):
         ab = ("hAB[%d]" % d) if sign > 0 else ("(-hAB[%d])" % d)
         w(ind + "    const double %s = fma(%s, %s, %s);" % (nm, ab, bval(s1), bval(s0)))
     for ab in bra_final:
-        w(ind + "    abf[%d * NF + f] = %s;" % (abidx[ab], bval(bra_map[ab])))
+        w(ind + "    abf[(%d * NF + f) * B + b] = %s;" % (abidx[ab], bval(bra_map[ab])))
     w(ind + "}")
     w(ind + "__syncthreads();")
-    # ---- ket HRR: one lane per final bra component
+    w(ind + "if (b < nb) {")
     if g.ket_hrr_needed():
-        w(ind + "const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
-    w(ind + "double *__restrict__ o = A.out + wi.obase + A.ket.col[tl] * A.stride;")
-    w(ind + "for (int ab = lane; ab < NAB; ab += 64) {")
-    w(ind + "    const double *R = abf + ab * NF;")
-    w(ind + "    double *oo = o + ab * NCD;")
+        w(ind + "    const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
+    w(ind + "    double *__restrict__ o = A.out + wi.obase + A.ket.col[tl] * A.stride;")
+    w(ind + "    for (int ab = r0; ab < NAB; ab += LPB) {")
+    w(ind + "        const double *R = abf + ab * NF * B + b;")
+    w(ind + "        double *oo = o + ab * NCD;")
     knames = {}
     cnt = [0]
 
     def kval(src):
         if src[0] == "top":
-            return "R[%d]" % ftop[src[1]]
+            return "R[%d * B]" % ftop[src[1]]
         return knames[(src[1], src[2])]
 
     for (dst, s0, s1, sign, d) in ket_ops:
@@ -281,10 +340,11 @@ def _emit_hrr_coop(g, w, am):
         nm = "hk%d" % cnt[0]
         knames[(dst[1], dst[2])] = nm
         cd = ("hCD[%d]" % d) if sign > 0 else ("(-hCD[%d])" % d)
-        w(ind + "    const double %s = fma(%s, %s, %s);" % (nm, cd, kval(s1), kval(s0)))
+        w(ind + "        const double %s = fma(%s, %s, %s);" % (nm, cd, kval(s1), kval(s0)))
     k = 0
     for c in carts(lc):
         for dd in carts(ld):
-            w(ind + "    oo[%d] = %s;" % (k, kval(ket_map[(c, dd)])))
+            w(ind + "        oo[%d] = %s;" % (k, kval(ket_map[(c, dd)])))
             k += 1
+    w(ind + "    }")
     w(ind + "}")

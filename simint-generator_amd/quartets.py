@@ -1,0 +1,171 @@
+"""Screened shell-quartet sets on the device: the caller side of the hot path.
+
+Builds, for a list of shells, every unique shell quartet (ij|kl) with
+i >= j, k >= l, ij >= kl whose Schwarz bound sqrt((ij|ij)) sqrt((kl|kl))
+reaches a threshold, and runs them through simint_hip_compute_eri_prefix.
+
+Layout.  Unique pairs are oriented so am1 >= am2 and grouped by (am1, am2)
+("pair class") and by primitive-pair count, so the 64 kets of a tile share
+one primitive loop.  Inside a group, pairs are sorted by their bound (the
+largest diagonal element of (ij|ij), computed on the GPU from the diagonal
+quartets, shell_screen.c:9-60), descending: the kets that survive the screen
+for a given bra are then a prefix of each group.  Quartets between groups
+are counted once: a bra meets ket groups of lower (class, group) keys fully
+and its own group up to itself.
+
+Everything a pass needs is resident in HBM; a pass only uploads the per-bra
+prefix lengths and output offsets of each launch.
+"""
+import ctypes
+
+import numpy as np
+
+import simint
+
+
+class Launch:
+    __slots__ = ("bra", "ket", "b0", "nket", "off", "nq", "flops", "nout", "cls")
+
+    def __init__(self, bra, ket, b0, nket, off, nq, flops, nout, cls):
+        self.bra, self.ket, self.b0, self.nket, self.off = bra, ket, b0, nket, off
+        self.nq, self.flops, self.nout, self.cls = nq, flops, nout, cls
+
+
+class Group:
+    """One (pair class, primitive count) group of unique pairs, sorted by bound."""
+
+    def __init__(self, basis, cls, nprim, pairs, bounds):
+        self.cls = cls
+        self.nprim = nprim
+        self.pairs = pairs
+        self.bounds = bounds
+        self.host = simint.Pairs.from_pairs(basis, pairs)
+        self.dev = simint.DevicePairs(self.host)
+
+
+def _class_flops(info, la, lb, lc, ld):
+    c = info[simint_class_name((la, lb, lc, ld))]
+    per_prim = c["f_setup"] + c["f_vrr"] + c["f_boys"] + c["bcont"]
+    return per_prim, c["f_hrr"], c["ncart"]
+
+
+def simint_class_name(am):
+    return "_".join("spdfghik"[x] for x in am)
+
+
+class ScreenedSet:
+    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29):
+        import torch
+        self.tol = tol
+        self.basis = simint.Basis(shells)
+        self.shells = shells
+        self.out_doubles = out_doubles
+        L = simint.lib()
+        L.simint_init()
+        # unique pairs by class
+        byclass = {}
+        for i in range(len(shells)):
+            for j in range(i + 1):
+                a, b = (i, j) if shells[i][0] >= shells[j][0] else (j, i)
+                byclass.setdefault((shells[a][0], shells[b][0]), []).append((a, b))
+        # Schwarz bounds from diagonal quartets on the GPU
+        self.groups = []
+        for cls in sorted(byclass):
+            pairs = byclass[cls]
+            tmp = simint.Pairs.from_pairs(self.basis, pairs)
+            dev = simint.DevicePairs(tmp)
+            nab = simint.ncart(cls[0]) * simint.ncart(cls[1])
+            out = torch.empty(len(pairs) * nab * nab, dtype=torch.float64, device="cuda")
+            simint.compute_diag(dev, out.data_ptr())
+            simint.synchronize()
+            diag = out.view(len(pairs), nab, nab).diagonal(dim1=1, dim2=2).abs().amax(dim=1).cpu().numpy()
+            nprim = tmp.field("nprim12")
+            dev.free()
+            tmp.free()
+            for np_ in sorted(set(nprim.tolist()), reverse=True):
+                sel = np.nonzero(nprim == np_)[0]
+                order = sel[np.argsort(-diag[sel], kind="stable")]
+                self.groups.append(Group(self.basis, cls, int(np_), [pairs[k] for k in order], diag[order].copy()))
+        import json
+        import os
+        here = os.path.dirname(os.path.abspath(__file__))
+        self.info = json.load(open(os.path.join(here, "csrc", "gen", "classes.json")))
+        self._plan()
+
+    def _plan(self):
+        """Per (bra group, ket group): prefix lengths for every bra, chunked
+        so that one launch's output fits the output buffer."""
+        tol2 = self.tol * self.tol
+        self.launches = []
+        self.total_quartets = 0
+        self.total_flops = 0.0
+        self.total_bytes = 0.0
+        G = self.groups
+        for gi, gb in enumerate(G):
+            for gk in G[:gi + 1]:
+                la, lb = gb.cls
+                lc, ld = gk.cls
+                pp, fh, nc = _class_flops(self.info, la, lb, lc, ld)
+                # kets surviving the screen: bound_b * bound_k >= tol^2
+                kb = gk.bounds
+                with np.errstate(divide="ignore"):
+                    need = np.where(gb.bounds > 0, tol2 / gb.bounds, np.inf)
+                # kb is descending: count of kb >= need
+                nket = np.searchsorted(-kb, -need, side="right").astype(np.int64)
+                if gk is gb:
+                    nket = np.minimum(nket, np.arange(1, len(gb.bounds) + 1))
+                K = gb.nprim * gk.nprim
+                # chunk bras so output per launch <= out_doubles
+                b = 0
+                nb = len(nket)
+                while b < nb:
+                    cum = np.cumsum(nket[b:] * nc)
+                    e = b + max(1, int(np.searchsorted(cum, self.out_doubles, side="right")))
+                    e = min(e, nb)
+                    seg = nket[b:e]
+                    nq = int(seg.sum())
+                    if nq > 0:
+                        off = np.zeros(len(seg), dtype=np.int64)
+                        off[1:] = np.cumsum(seg[:-1] * nc)
+                        fl = float(nq) * (K * pp + fh)
+                        self.launches.append(Launch(gb, gk, b, seg.astype(np.int32), off, nq, fl, nq * nc,
+                                                    (la, lb, lc, ld)))
+                        self.total_quartets += nq
+                        self.total_flops += fl
+                        self.total_bytes += 8.0 * nq * nc
+                    b = e
+
+    def run_pass(self, out_ptr, events=None):
+        """Queue every launch of one pass on the library stream.  If events is
+        a callable, it is called before and after each launch with the launch
+        (to record timing events on the library stream)."""
+        for ln in self.launches:
+            if events:
+                events(ln, True)
+            simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out_ptr, 0.0)
+            if events:
+                events(ln, False)
+
+    def sample(self, n, seed=0):
+        """n random quartets of the screened set, as shell-index quadruples
+        with their class, weighted uniformly over quartets."""
+        rng = np.random.default_rng(seed)
+        w = np.array([ln.nq for ln in self.launches], dtype=np.float64)
+        picks = rng.choice(len(self.launches), size=n, p=w / w.sum())
+        out = []
+        for li in picks:
+            ln = self.launches[li]
+            cum = np.cumsum(ln.nket)
+            r = int(rng.integers(0, cum[-1]))
+            bi = int(np.searchsorted(cum, r, side="right"))
+            k = r - (cum[bi - 1] if bi > 0 else 0)
+            bp = ln.bra.pairs[ln.b0 + bi]
+            kp = ln.ket.pairs[k]
+            out.append((bp[0], bp[1], kp[0], kp[1]))
+        return out
+
+    def free(self):
+        for g in self.groups:
+            g.dev.free()
+            g.host.free()
+        self.basis.free()
