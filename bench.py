@@ -34,6 +34,23 @@ PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` as recorded in the rocprofv3 summaries
+    under profiles/ (FETCH_SIZE + WRITE_SIZE, calibrated against a copy
+    kernel), or None if that kernel has not been measured."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for f in sorted(os.listdir(pdir)):
+        if not f.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, f)):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def cpu_baseline(ss, seconds=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -154,7 +171,10 @@ def main():
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": achieved / (PEAK_HBM / 1e9), "traffic": None}
     info = ss.info[quartets.simint_class_name(dom)]
-    roof.update({"kernel": "eri_%s_%s" % (info["kernel"], quartets.simint_class_name(dom)),
+    kname = "eri_%s_%s" % (info["kernel"], quartets.simint_class_name(dom))
+    roof["traffic"] = measured_traffic(kname)
+    roof.update({"kernel": kname,
+                 "algorithmic_bytes_per_launch": bytes_per_launch,
                  "compute": "FP64 VALU (no MFMA); peak is the FP64 vector rate",
                  "launches": d["launches"], "mean_launch_ms": 1000.0 * mean_s,
                  "flops_per_launch": flops_per_launch, "share_of_device_time":

@@ -1,0 +1,224 @@
+"""Pins the CPU oracle (oracle/oracle.c) -- CPU only.
+
+What fixes each part:
+  * Boys grid / long-range factors: mpmath (gammainc), closed forms, and the
+    reference's own tables compiled from its sources (oracle/_ref) where built;
+  * Boys split scheme: mpmath, with the asymptotic-branch error bound the
+    reference's scheme carries near its cutoff;
+  * (ss|ss): closed form 2 pi^(5/2) / (p q sqrt(p+q)) K_AB K_CD F_0(T);
+  * every class up to f: an independent long-double McMurchie-Davidson brute
+    force (oracle/md_ref.c), fed the reference's Boys values so that the
+    recurrence algebra is compared exactly, and fed exact Boys values with
+    the Boys-scheme error allowance;
+  * invariants: bra/ket and centre permutations, translation, Schwarz bound;
+  * first derivatives: Richardson-extrapolated finite differences of the
+    long-double brute force.
+"""
+import ctypes
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import workloads
+from helpers import mol_shells, normalized_arrays, per_quartet_errors
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_TABLES = os.path.join(ROOT, "oracle", "_ref", "libsimint_ref_tables.so")
+PRODUCT_GRID = os.path.join(ROOT, "simint-generator_amd", "csrc", "gen", "boys_grid.inc")
+
+
+def _mp_boys(n, x):
+    from mpmath import mp
+    mp.dps = 40
+    x = mp.mpf(x)
+    if x == 0:
+        return mp.mpf(1) / (2 * n + 1)
+    N = n + mp.mpf("0.5")
+    return mp.gammainc(N, 0, x) / (2 * mp.power(x, N))
+
+
+def test_boys_grid_against_mpmath():
+    g = oracle.boys_grid()
+    assert np.all(g[0] == 1.0 / (2 * np.arange(32) + 1))
+    from mpmath import mp
+    for i in (1, 7, 50, 123, 250, 365):
+        for m in (0, 1, 5, 13, 31):
+            v = float(_mp_boys(m, mp.mpf(i) / 10))     # grid point x_i = i/10 exactly
+            assert abs(g[i, m] - v) <= 1.2e-16 * abs(v), (i, m)
+
+
+def test_boys_longfac_closed_form():
+    lf = oracle.boys_longfac()
+    assert lf[0] == pytest.approx(math.sqrt(math.pi) / 2, rel=1e-16)
+    for n in range(1, 32):
+        assert lf[n] == pytest.approx(lf[n - 1] * (2 * n - 1) / 2, rel=4e-16)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TABLES), reason="reference tables not built here")
+def test_tables_match_reference_sources():
+    L = ctypes.CDLL(REF_TABLES)
+    ref = np.ctypeslib.as_array((ctypes.c_double * (366 * 32)).in_dll(L, "boys_shortgrid")).reshape(366, 32)
+    g = oracle.boys_grid()
+    ulp = np.spacing(np.abs(ref))
+    assert np.all(np.abs(g - ref) <= ulp), np.abs(g - ref).max()
+    L.simint_ref_boys_longfac.restype = ctypes.POINTER(ctypes.c_double)
+    lf = np.ctypeslib.as_array(L.simint_ref_boys_longfac(), shape=(32,))
+    assert np.all(np.abs(oracle.boys_longfac() - lf) <= np.spacing(lf))
+    nf = np.ctypeslib.as_array((ctypes.c_double * 21).in_dll(L, "norm_fac"))
+    for l in range(21):
+        assert abs(oracle.lib().oracle_norm_fac(l) - nf[l]) <= np.spacing(nf[l])
+
+
+@pytest.mark.skipif(not os.path.exists(PRODUCT_GRID), reason="generated sources not present")
+def test_product_grid_matches_oracle_grid():
+    import re
+    text = re.sub(r"/\*.*?\*/", "", open(PRODUCT_GRID).read(), flags=re.S)
+    vals = [float.fromhex(t) for t in text.replace("\n", " ").split(",") if t.strip()]
+    g = np.array(vals).reshape(366, 32)
+    o = oracle.boys_grid()
+    assert np.all(np.abs(g - o) <= np.spacing(np.abs(o)))
+
+
+@pytest.mark.parametrize("x", [0.0, 1e-9, 0.04999, 0.05, 3.3333, 17.2, 30.0, 36.49, 36.5, 40.0, 120.0, 5e3])
+def test_boys_split_scheme(x):
+    for L in (0, 2, 3, 4, 8, 12):
+        F = oracle.boys(x, L)
+        for m in range(L + 1):
+            exact = float(_mp_boys(m, x))
+            if x < 36.5:
+                tol = 2e-14
+            else:
+                # the asymptotic form drops Gamma(m+1/2, x)/Gamma(m+1/2)
+                from mpmath import mp
+                mp.dps = 40
+                tol = float(mp.gammainc(m + mp.mpf("0.5"), x) / mp.gamma(m + mp.mpf("0.5"))) * 4 + 2e-14
+                if L >= 4:
+                    tol = max(tol, float(mp.gammainc(L + mp.mpf("0.5"), x) / mp.gamma(L + mp.mpf("0.5"))) * 4)
+            assert abs(F[m] - exact) <= tol * exact, (x, L, m, F[m], exact)
+
+
+def test_ssss_closed_form():
+    rng = np.random.default_rng(4)
+    shells = []
+    for _ in range(6):
+        shells.append((0, tuple(rng.uniform(-3, 3, 3)), [float(rng.uniform(0.1, 30))], [1.0]))
+    sa = normalized_arrays(shells)
+    for q in itertools.combinations_with_replacement(range(6), 4):
+        got, _, _ = oracle.eri(sa, [q])
+        a, b, c, d = [shells[i] for i in q]
+        ca, cb, cc, cd = [sa.coef[sa.off[i]] for i in q]
+        A, B, C, D = [np.array(s[1]) for s in (a, b, c, d)]
+        ea, eb, ec, ed = a[2][0], b[2][0], c[2][0], d[2][0]
+        p, qq = ea + eb, ec + ed
+        P, Q = (ea * A + eb * B) / p, (ec * C + ed * D) / qq
+        T = p * qq / (p + qq) * np.sum((P - Q) ** 2)
+        kab = math.exp(-ea * eb / p * np.sum((A - B) ** 2))
+        kcd = math.exp(-ec * ed / qq * np.sum((C - D) ** 2))
+        want = 2 * math.pi ** 2.5 / (p * qq * math.sqrt(p + qq)) * kab * kcd * float(_mp_boys(0, T)) * ca * cb * cc * cd
+        # exp() turns the rounding of its argument into a relative error of
+        # |argument| ulps, so the bound grows with the exponents' arguments
+        args = ea * eb / p * np.sum((A - B) ** 2) + ec * ed / qq * np.sum((C - D) ** 2) + T
+        assert abs(got[0] - want) <= (2e-15 + 4e-16 * args) * abs(want) + 1e-300
+
+
+def _cases():
+    out = []
+    shells = mol_shells("water.6-31gss")
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    rng = np.random.default_rng(1)
+    qs = []
+    for am in itertools.product(sorted(byam), repeat=4):
+        for _ in range(2):
+            qs.append(tuple(int(rng.choice(byam[a])) for a in am))
+    out.append(("water.6-31gss", shells, qs))
+    shells = mol_shells("water.aug-cc-pvtz")
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    qs = []
+    for am in itertools.product(sorted(byam), repeat=4):
+        if sum(am) > 9:
+            continue
+        qs.append(tuple(int(rng.choice(byam[a])) for a in am))
+    out.append(("water.aug-cc-pvtz (to f)", shells, qs))
+    shells = workloads.config2(natom=2, seed=7)
+    qs = [tuple(int(x) for x in rng.integers(0, len(shells), 4)) for _ in range(300)]
+    out.append(("config2 2 atoms", shells, qs))
+    shells, qs1 = workloads.config1()
+    out.append(("config1", shells, qs1[:300]))
+    return out
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c[0])
+def test_oracle_matches_brute_force(case):
+    _, shells, qs = case
+    sa = normalized_arrays(shells)
+    o, offs, sizes = oracle.eri(sa, qs)
+    m, _, _ = oracle.md_eri(sa, qs, reference_boys=True)
+    d = np.abs(o - m)
+    # same Boys values: only FP64 rounding of the recurrences separates them
+    assert np.all(d <= 1e-14 * np.maximum(1.0, np.abs(m))), d.max()
+    e = per_quartet_errors(o, m, offs, sizes)
+    assert np.median(e) < 1e-14
+    # exact Boys values: the reference scheme's own approximation error
+    mx, _, _ = oracle.md_eri(sa, qs)
+    assert np.all(np.abs(o - mx) <= 1e-12 * np.maximum(1.0, np.abs(mx)))
+
+
+def _block(out, offs, sizes, k, shape):
+    return out[offs[k]:offs[k] + sizes[k]].reshape(shape)
+
+
+def test_invariants():
+    shells = workloads.config2(natom=2, seed=3)
+    sa = normalized_arrays(shells)
+    rng = np.random.default_rng(2)
+    nc = [(s[0] + 1) * (s[0] + 2) // 2 for s in shells]
+    for _ in range(40):
+        a, b, c, d = (int(x) for x in rng.integers(0, len(shells), 4))
+        qs = [(a, b, c, d), (b, a, c, d), (a, b, d, c), (c, d, a, b)]
+        o, offs, sizes = oracle.eri(sa, qs)
+        sh = (nc[a], nc[b], nc[c], nc[d])
+        x0 = _block(o, offs, sizes, 0, sh)
+        x1 = _block(o, offs, sizes, 1, (nc[b], nc[a], nc[c], nc[d])).transpose(1, 0, 2, 3)
+        x2 = _block(o, offs, sizes, 2, (nc[a], nc[b], nc[d], nc[c])).transpose(0, 1, 3, 2)
+        x3 = _block(o, offs, sizes, 3, (nc[c], nc[d], nc[a], nc[b])).transpose(2, 3, 0, 1)
+        s = max(1.0, np.abs(x0).max())
+        for x in (x1, x2, x3):
+            assert np.abs(x - x0).max() <= 1e-14 * s
+        # Schwarz: |(ab|cd)| <= sqrt((ab|ab)) sqrt((cd|cd)) element-wise on the diagonal blocks
+        d1, o1, s1 = oracle.eri(sa, [(a, b, a, b), (c, d, c, d)])
+        gab = _block(d1, o1, s1, 0, (nc[a] * nc[b], nc[a] * nc[b])).diagonal()
+        gcd = _block(d1, o1, s1, 1, (nc[c] * nc[d], nc[c] * nc[d])).diagonal()
+        bound = np.sqrt(np.outer(np.abs(gab), np.abs(gcd))).reshape(sh)
+        assert np.all(np.abs(x0) <= bound * (1 + 1e-12) + 1e-300)
+    # translation of every centre by the same vector
+    moved = [(s[0], tuple(np.array(s[1]) + np.array([1.25, -0.5, 3.0])), s[2], s[3]) for s in shells]
+    sb = normalized_arrays(moved)
+    qs = [tuple(int(x) for x in rng.integers(0, len(shells), 4)) for _ in range(30)]
+    o, offs, sizes = oracle.eri(sa, qs)
+    t, _, _ = oracle.eri(sb, qs)
+    assert np.all(np.abs(o - t) <= 1e-12 * np.maximum(1.0, np.abs(o)))
+
+
+def test_deriv1_against_finite_differences():
+    shells = workloads.config2(natom=2, seed=11)
+    sa = normalized_arrays(shells)
+    rng = np.random.default_rng(5)
+    qs = [tuple(int(x) for x in rng.integers(0, len(shells), 4)) for _ in range(25)]
+    qs += [(0, 0, 3, 3), (5, 5, 5, 5), (2, 4, 2, 4)]
+    g, offs, sizes = oracle.eri(sa, qs, deriv=1)
+    f, _, _ = oracle.md_eri_deriv1(sa, qs, h=1e-4)
+    for k in range(len(qs)):
+        a = g[offs[k]:offs[k] + sizes[k]]
+        b = f[offs[k]:offs[k] + sizes[k]]
+        scale = np.abs(b).max()
+        # finite differences carry ~1e-9 relative error; components that
+        # vanish by symmetry are compared absolutely
+        assert np.abs(a - b).max() <= 1e-8 * scale + 1e-14, (qs[k], np.abs(a - b).max(), scale)
