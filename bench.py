@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-class", default=None,
                     help="profiling aid: run only this class's launches (e.g. d_p_d_p)")
+    ap.add_argument("--force-family", choices=["lpq", "wpq"], default=None,
+                    help="profiling aid: use this kernel family for every class that has it")
     ap.add_argument("--copy-gib", type=float, default=None,
                     help="profiling aid: run one device copy of this many GiB and exit "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -109,6 +111,13 @@ def main():
     import workloads
 
     shells = workloads.config2(natom=args.natom, seed=0)
+    if args.force_family:
+        fam = {"lpq": 1, "wpq": 2}[args.force_family]
+        for a in range(3):
+            for b in range(3):
+                for c in range(3):
+                    for d in range(3):
+                        simint.lib().simint_hip_set_class_kernel(a, b, c, d, fam)
     t_setup = time.perf_counter()
     ss = quartets.ScreenedSet(shells, tol=1e-12)
     t_setup = time.perf_counter() - t_setup

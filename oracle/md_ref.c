@@ -83,14 +83,26 @@ static void cart_list(int l, int (*e)[3])
         for (int iy = l - ix; iy >= 0; iy--) { e[n][0] = ix; e[n][1] = iy; e[n][2] = l - ix - iy; n++; }
 }
 
+/* position of x^ix y^iy z^iz in the order cart_list writes */
+static inline int cart_index(int ix, int iy, int iz)
+{
+    const int l = ix + iy + iz;
+    return (l - ix) * (l - ix + 1) / 2 + (l - ix - iy);
+}
+
 typedef struct {
     int am, nprim;
     ld c[3];
     const double *alpha, *coef;
 } md_shell;
 
-/* contracted (ab|cd) for one quartet, accumulated into out (long double) */
-static void md_quartet(const md_shell *S[4], ld *out)
+/* contracted (ab|cd) for one quartet, accumulated into out (long double).
+ * wcen >= 0: each primitive quartet is weighted by 2 * (exponent of the
+ * primitive on centre wcen), the factor of the analytic derivative
+ * d/dA_x G_a = 2 alpha G_{a+1_x} - a_x G_{a-1_x}. */
+static int g_absmode = 0;   /* 1: accumulate |primitive contribution| */
+
+static void md_quartet_w(const md_shell *S[4], int wcen, ld *out)
 {
     const int la = S[0]->am, lb = S[1]->am, lc = S[2]->am, ld_ = S[3]->am;
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld_);
@@ -123,7 +135,11 @@ static void md_quartet(const md_shell *S[4], ld *out)
                         RPQ2 += (P[x] - Q[x]) * (P[x] - Q[x]);
                     }
                     const ld alpha = p * q / (p + q);
-                    const ld cc = (ld)S[0]->coef[i] * S[1]->coef[j] * S[2]->coef[k] * S[3]->coef[l];
+                    ld cc = (ld)S[0]->coef[i] * S[1]->coef[j] * S[2]->coef[k] * S[3]->coef[l];
+                    if (wcen >= 0) {
+                        const ld ex[4] = {a, b, c, d};
+                        cc *= 2.0L * ex[wcen];
+                    }
                     const ld pref = cc * 2.0L * powl(pi, 2.5L) / (p * q * sqrtl(p + q))
                                     * expl(-a * b / p * AB2) * expl(-c * d / q * CD2);
                     for (int x = 0; x < 3; x++) {
@@ -192,12 +208,14 @@ static void md_quartet(const md_shell *S[4], ld *out)
                                             }
                                         }
                                     }
-                                    out[idx] += pref * s;
+                                    out[idx] += g_absmode ? fabsl(pref * s) : pref * s;
                                 }
                 }
     free(Rn); free(Rn1);
 #undef RI
 }
+
+static void md_quartet(const md_shell *S[4], ld *out) { md_quartet_w(S, -1, out); }
 
 static void load(int nsh, const int *am, const int *nprim, const int *prim_off, const double *xyz,
                  const double *alpha, const double *coef, md_shell *s)
@@ -276,6 +294,91 @@ int md_eri_deriv1_quartets(int nsh, const int *am, const int *nprim, const int *
             }
         free(p1); free(m1); free(p2); free(m2);
     }
+    free(s);
+    return 0;
+}
+
+/* first derivatives from the analytic formula, every term in long double:
+ * d/dX (..a..) = sum over primitives [2 alpha (..a+1_x..) - a_x (..a-1_x..)]
+ * for all four centres (no translational-invariance shortcut).  Same layout
+ * as md_eri_deriv1_quartets.  This is the exact value of what simint's
+ * first-derivative kernels compute (for a pair built from one shell, the
+ * symmetrised weights alpha_i + alpha_j of shell.c:512-526 give the same sum,
+ * the primitive pair being symmetric on one centre). */
+int md_eri_deriv1_terms(int nsh, const int *am, const int *nprim, const int *prim_off,
+                        const double *xyz, const double *alpha, const double *coef,
+                        int nq, const int *quartets, const long long *offsets, int scale, double *out);
+
+int md_eri_deriv1_analytic_quartets(int nsh, const int *am, const int *nprim, const int *prim_off,
+                                    const double *xyz, const double *alpha, const double *coef,
+                                    int nq, const int *quartets, const long long *offsets, double *out)
+{
+    return md_eri_deriv1_terms(nsh, am, nprim, prim_off, xyz, alpha, coef, nq, quartets, offsets, 0, out);
+}
+
+/* scale: 1 = instead of the derivative, the sum over primitives of the
+ * magnitudes of both terms, |2 alpha (a+1)| + a_x |(a-1)|, i.e. the size of
+ * what the derivative is formed from; FP64 evaluations of the formula can
+ * differ by a few ulps of this, not of the (possibly much smaller) result. */
+int md_eri_deriv1_terms(int nsh, const int *am, const int *nprim, const int *prim_off,
+                        const double *xyz, const double *alpha, const double *coef,
+                        int nq, const int *quartets, const long long *offsets, int scale, double *out)
+{
+    g_absmode = scale;
+    md_shell *s = (md_shell *)malloc(sizeof(md_shell) * (size_t)nsh);
+    load(nsh, am, nprim, prim_off, xyz, alpha, coef, s);
+    for (int n = 0; n < nq; n++) {
+        const int *q = quartets + 4 * n;
+        md_shell sh[4];
+        for (int c = 0; c < 4; c++) sh[c] = s[q[c]];
+        if (sh[0].am + 1 > MD_MAXL || sh[1].am + 1 > MD_MAXL || sh[2].am + 1 > MD_MAXL || sh[3].am + 1 > MD_MAXL) {
+            free(s);
+            return -1;
+        }
+        const int n4[4] = {ncart(sh[0].am), ncart(sh[1].am), ncart(sh[2].am), ncart(sh[3].am)};
+        const int nall = n4[0] * n4[1] * n4[2] * n4[3];
+        int e4[4][MD_MAXL * 8][3];
+        for (int c = 0; c < 4; c++) cart_list(sh[c].am, e4[c]);
+        for (int cen = 0; cen < 4; cen++) {
+            md_shell sp[4] = {sh[0], sh[1], sh[2], sh[3]}, sm[4] = {sh[0], sh[1], sh[2], sh[3]};
+            sp[cen].am += 1;
+            sm[cen].am -= 1;
+            const md_shell *Sp[4] = {&sp[0], &sp[1], &sp[2], &sp[3]};
+            const md_shell *Sm[4] = {&sm[0], &sm[1], &sm[2], &sm[3]};
+            int np4[4] = {n4[0], n4[1], n4[2], n4[3]}, nm4[4] = {n4[0], n4[1], n4[2], n4[3]};
+            np4[cen] = ncart(sp[cen].am);
+            ld *plus = (ld *)malloc(sizeof(ld) * (size_t)np4[0] * np4[1] * np4[2] * np4[3]);
+            md_quartet_w(Sp, cen, plus);
+            ld *minus = NULL;
+            if (sh[cen].am > 0) {
+                nm4[cen] = ncart(sm[cen].am);
+                minus = (ld *)malloc(sizeof(ld) * (size_t)nm4[0] * nm4[1] * nm4[2] * nm4[3]);
+                md_quartet(Sm, minus);
+            }
+            for (int k = 0; k < nall; k++) {
+                int comp[4], r = k;
+                for (int c = 3; c >= 0; c--) { comp[c] = r % n4[c]; r /= n4[c]; }
+                for (int x = 0; x < 3; x++) {
+                    int e[3] = {e4[cen][comp[cen]][0], e4[cen][comp[cen]][1], e4[cen][comp[cen]][2]};
+                    int cp[4] = {comp[0], comp[1], comp[2], comp[3]};
+                    e[x] += 1;
+                    cp[cen] = cart_index(e[0], e[1], e[2]);
+                    ld v = plus[((cp[0] * np4[1] + cp[1]) * np4[2] + cp[2]) * np4[3] + cp[3]];
+                    e[x] -= 2;
+                    if (e[x] >= 0) {
+                        int cm[4] = {comp[0], comp[1], comp[2], comp[3]};
+                        cm[cen] = cart_index(e[0], e[1], e[2]);
+                        const ld mv = (ld)(e[x] + 1) * minus[((cm[0] * nm4[1] + cm[1]) * nm4[2] + cm[2]) * nm4[3] + cm[3]];
+                        v = scale ? v + mv : v - mv;
+                    }
+                    out[offsets[n] + (long long)k * 12 + cen * 3 + x] = (double)v;
+                }
+            }
+            free(plus);
+            free(minus);
+        }
+    }
+    g_absmode = 0;
     free(s);
     return 0;
 }

@@ -28,6 +28,8 @@ def lib():
         L.md_eri_quartets.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp, dp, ctypes.c_int, ip, llp, dp]
         L.md_eri_deriv1_quartets.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp, dp, ctypes.c_int, ip, llp,
                                              ctypes.c_double, dp]
+        L.md_eri_deriv1_terms.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp, dp, ctypes.c_int, ip, llp,
+                                          ctypes.c_int, dp]
         L.oracle_fill_pair.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, dp, dp]
         L.oracle_normalize.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp]
@@ -117,6 +119,20 @@ def md_eri(sa, quartets, reference_boys=False):
     lib().md_set_reference_boys(0)
     if rc != 0:
         raise ValueError("md_eri: angular momentum too high")
+    return out, off, n
+
+
+def md_eri_deriv1_analytic(sa, quartets, scale=False):
+    """Long-double first derivatives from 2a (a+1) - a_x (a-1) on every
+    centre, McMurchie-Davidson integrals (exact Boys values).  scale=True
+    returns instead, per element, the summed magnitudes of the primitive
+    terms the derivative is formed from (its conditioning scale)."""
+    q, n, off = quartet_sizes(sa, quartets, 1)
+    out = np.zeros(int(n.sum()), dtype=np.float64)
+    rc = lib().md_eri_deriv1_terms(*sa.args(), len(q), _p(q, ctypes.c_int), _p(off, ctypes.c_longlong),
+                                   1 if scale else 0, _p(out, ctypes.c_double))
+    if rc != 0:
+        raise ValueError("md_eri_deriv1_analytic: angular momentum too high")
     return out, off, n
 
 

@@ -26,7 +26,7 @@ GEN = os.path.join(CSRC, "gen")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libsimint.so")
 
-DEFAULT_MAXAM = 2
+DEFAULT_MAXAM = 3
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-I" + INCLUDE, "-I" + CSRC,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]

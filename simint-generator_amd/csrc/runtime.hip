@@ -145,16 +145,22 @@ struct Image {
     template <class T> T *at(size_t off) { return reinterpret_cast<T *>(buf.data() + off); }
 };
 
+// prefac_a2 / prefac_b2 (when present) hold prefac * alpha2 and prefac *
+// beta2: with them an ordinary class kernel of class (a+1 b|..) yields the
+// 2a-weighted contraction a first derivative needs
+// (OSTEIDeriv1_Writer.cpp:745-746, :800-801 contract with these factors).
 struct LinearOffs {
     size_t alpha, inv, prefac, x, y, z, pax, pay, paz, pbx, pby, pbz, screen, a2, b2, poff, np, abx, aby, abz;
+    size_t prefac_a2 = 0, prefac_b2 = 0;
 };
 struct TiledOffs {
     size_t alpha, inv, prefac, x, y, z, pax, pay, paz, pbx, pby, pbz, screen, a2, b2, toff, tnp, abx, aby, abz, col;
+    size_t prefac_a2 = 0, prefac_b2 = 0;
     int ntile;
 };
 
 // bra view: pairs [0, n) of P, primitives contiguous as in the host struct
-LinearOffs pack_linear(Image &im, const simint_multi_shellpair *P, int n)
+LinearOffs pack_linear(Image &im, const simint_multi_shellpair *P, int n, bool weighted = false)
 {
     LinearOffs o;
     long long nprim = 0;
@@ -178,6 +184,15 @@ LinearOffs pack_linear(Image &im, const simint_multi_shellpair *P, int n)
     double *inv = im.at<double>(o.inv);
     const double *al = P->alpha;
     for (long long k = 0; k < nprim; k++) inv[k] = 1.0 / al[k];
+    if (weighted) {
+        o.prefac_a2 = im.add(db);
+        o.prefac_b2 = im.add(db);
+        double *wa = im.at<double>(o.prefac_a2), *wb = im.at<double>(o.prefac_b2);
+        for (long long k = 0; k < nprim; k++) {
+            wa[k] = P->prefac[k] * P->alpha2[k];
+            wb[k] = P->prefac[k] * P->beta2[k];
+        }
+    }
     int *poff = im.at<int>(o.poff), *np = im.at<int>(o.np);
     int acc = 0;
     for (int i = 0; i < n; i++) {
@@ -194,7 +209,7 @@ LinearOffs pack_linear(Image &im, const simint_multi_shellpair *P, int n)
 // ket view: pairs [0, n) of Q in tiles of 64.  `order` lists the pairs in
 // tile order; col[] maps a tile lane back to its position in `colmap`.
 TiledOffs pack_tiled(Image &im, const simint_multi_shellpair *Q, const std::vector<int> &order,
-                     const std::vector<long long> &colmap)
+                     const std::vector<long long> &colmap, bool weighted = false)
 {
     TiledOffs o;
     const int n = (int)order.size();
@@ -238,6 +253,21 @@ TiledOffs pack_tiled(Image &im, const simint_multi_shellpair *Q, const std::vect
         f[13][k] = 1.0;
         f[14][k] = 1.0;
     }
+    double *wa = nullptr, *wb = nullptr;
+    if (weighted) {
+        o.prefac_a2 = im.add(db);
+        o.prefac_b2 = im.add(db);
+        wa = im.at<double>(o.prefac_a2);
+        wb = im.at<double>(o.prefac_b2);
+        memset(wa, 0, db);
+        memset(wb, 0, db);
+        // f[] pointers may have moved with the buffer
+        double *g[15] = {im.at<double>(o.alpha), im.at<double>(o.inv), im.at<double>(o.prefac), im.at<double>(o.x),
+                         im.at<double>(o.y), im.at<double>(o.z), im.at<double>(o.pax), im.at<double>(o.pay),
+                         im.at<double>(o.paz), im.at<double>(o.pbx), im.at<double>(o.pby), im.at<double>(o.pbz),
+                         im.at<double>(o.screen), im.at<double>(o.a2), im.at<double>(o.b2)};
+        for (int a = 0; a < 15; a++) f[a] = g[a];
+    }
     long long *colp = im.at<long long>(o.col);
     for (int t = 0; t < ntile; t++) {
         im.at<long long>(o.toff)[t] = toff[t];
@@ -262,6 +292,10 @@ TiledOffs pack_tiled(Image &im, const simint_multi_shellpair *Q, const std::vect
                     if (a == 1) f[a][k] = 1.0 / Q->alpha[s];
                     else if (src[a]) f[a][k] = src[a][s];
                     else f[a][k] = 0.0;
+                }
+                if (weighted) {
+                    wa[k] = Q->prefac[s] * Q->alpha2[s];
+                    wb[k] = Q->prefac[s] * Q->beta2[s];
                 }
             }
         }
@@ -309,6 +343,94 @@ int validate(const simint_multi_shellpair *P)
     return 0;
 }
 
+// ----------------------------------------------------- first derivatives
+// d/dX_i (ab|cd) for the three explicit centres from the 2a-weighted (a+1)
+// class and the (a-1) class, and the fourth centre (the first with the
+// highest AM) as minus their sum -- the assembly of
+// OSTEIDeriv1_Writer.cpp:334-537.  Output: 12 doubles per Cartesian,
+// [Ax Ay Az Bx ... Dz].
+struct Deriv1Args {
+    int am[4];
+    const double *plus[4];
+    const double *minus[4];
+    int skip;
+    long long nquartet;
+    double *out;
+};
+
+__device__ inline int dev_ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__device__ inline int dev_cart_index(int ix, int iy, int iz)
+{
+    const int l = ix + iy + iz;
+    return (l - ix) * (l - ix + 1) / 2 + (l - ix - iy);
+}
+
+__global__ void deriv1_assemble(const Deriv1Args d)
+{
+    const int n[4] = {dev_ncart(d.am[0]), dev_ncart(d.am[1]), dev_ncart(d.am[2]), dev_ncart(d.am[3])};
+    const long long nk = (long long)n[0] * n[1] * n[2] * n[3];
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= d.nquartet * nk) return;
+    const long long q = gid / nk;
+    int k = (int)(gid - q * nk);
+    int comp[4];
+    for (int c = 3; c >= 0; c--) {
+        comp[c] = k % n[c];
+        k /= n[c];
+    }
+    double res[12];
+    for (int c = 0; c < 4; c++) {
+        if (c == d.skip) continue;
+        // exponents of this centre's component (Psi4 order)
+        const int l = d.am[c];
+        int e[3] = {0, 0, 0};
+        {
+            int idx = 0;
+            for (int ix = l; ix >= 0; ix--)
+                for (int iy = l - ix; iy >= 0; iy--, idx++)
+                    if (idx == comp[c]) { e[0] = ix; e[1] = iy; e[2] = l - ix - iy; }
+        }
+        for (int x = 0; x < 3; x++) {
+            int ep[3] = {e[0], e[1], e[2]};
+            ep[x] += 1;
+            int dims[4] = {n[0], n[1], n[2], n[3]};
+            int cc[4] = {comp[0], comp[1], comp[2], comp[3]};
+            dims[c] = dev_ncart(l + 1);
+            cc[c] = dev_cart_index(ep[0], ep[1], ep[2]);
+            const long long np_ = (long long)dims[0] * dims[1] * dims[2] * dims[3];
+            const long long ip = ((cc[0] * dims[1] + cc[1]) * dims[2] + cc[2]) * dims[3] + cc[3];
+            double v = d.plus[c][q * np_ + ip];
+            if (e[x] > 0) {
+                int em[3] = {e[0], e[1], e[2]};
+                em[x] -= 1;
+                dims[c] = dev_ncart(l - 1);
+                cc[c] = dev_cart_index(em[0], em[1], em[2]);
+                const long long nm_ = (long long)dims[0] * dims[1] * dims[2] * dims[3];
+                const long long im_ = ((cc[0] * dims[1] + cc[1]) * dims[2] + cc[2]) * dims[3] + cc[3];
+                v -= e[x] * d.minus[c][q * nm_ + im_];
+            }
+            res[3 * c + x] = v;
+        }
+    }
+    for (int x = 0; x < 3; x++) {
+        double s = 0.0;
+        for (int c = 0; c < 4; c++)
+            if (c != d.skip) s -= res[3 * c + x];
+        res[3 * d.skip + x] = s;
+    }
+    double *o = d.out + gid * 12;
+    for (int t = 0; t < 12; t++) o[t] = res[t];
+}
+
+hipError_t launch_deriv1_assemble(const Deriv1Args &d, hipStream_t s)
+{
+    const long long nk = (long long)ncart(d.am[0]) * ncart(d.am[1]) * ncart(d.am[2]) * ncart(d.am[3]);
+    const long long n = d.nquartet * nk;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(deriv1_assemble, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C API
@@ -338,23 +460,54 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
     if (v) return v;
     v = validate(Q);
     if (v) return v;
-    if (deriv != 0) {
-        t_err = "derivative integrals are not built";
+    if (deriv < 0 || deriv > SIMINT_OSTEI_MAXDER) {
+        t_err = "derivative order not built";
         return SIMINT_ERROR_UNSUPPORTED;
     }
     // shell-level screen (ostei.c:17-20): screen values are not square-rooted
     const double tol2 = screen_tol * screen_tol;
     if (screen_tol > 0.0 && (P->screen_max * Q->screen_max) < tol2) return -1;
     if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
-    const int la = P->am1, lb = P->am2, lc = Q->am1, ld = Q->am2;
-    const simint_launch_fn fn = simint_dispatch(la, lb, lc, ld);
-    if (!fn) {
-        t_err = "class not built";
-        return SIMINT_ERROR_UNSUPPORTED;
-    }
+    const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
     const int nP = P->nshell12_clip, nQ = Q->nshell12_clip;
+    const int nc = ncart(am[0]) * ncart(am[1]) * ncart(am[2]) * ncart(am[3]);
+
+    // class launches: the class itself, or for deriv = 1 the (a+1) class of
+    // every centre but the one with the highest AM (2a-weighted) and the
+    // (a-1) class where a > 0 (ostei_deriv1_generator.cpp:96-124)
+    struct Term { int am[4]; int weight; int centre; int sign; simint_launch_fn fn; int nc; size_t off; };
+    std::vector<Term> terms;
+    int skip = -1;
+    if (deriv == 0) {
+        terms.push_back(Term{{am[0], am[1], am[2], am[3]}, 0, -1, 0, nullptr, nc, 0});
+    } else {
+        const int mx = std::max(std::max(am[0], am[1]), std::max(am[2], am[3]));
+        for (int c = 0; c < 4; c++)
+            if (am[c] == mx) { skip = c; break; }
+        for (int c = 0; c < 4; c++) {
+            if (c == skip) continue;
+            Term t{{am[0], am[1], am[2], am[3]}, 1 + (c & 1), c, +1, nullptr, 0, 0};
+            t.am[c] += 1;
+            terms.push_back(t);
+            if (am[c] > 0) {
+                Term m{{am[0], am[1], am[2], am[3]}, 0, c, -1, nullptr, 0, 0};
+                m.am[c] -= 1;
+                terms.push_back(m);
+            }
+        }
+    }
+    size_t scratch = 0;
+    for (auto &t : terms) {
+        if (t.am[0] > SIMINT_OSTEI_MAXAM || t.am[1] > SIMINT_OSTEI_MAXAM || t.am[2] > SIMINT_OSTEI_MAXAM ||
+            t.am[3] > SIMINT_OSTEI_MAXAM || !(t.fn = simint_dispatch(t.am[0], t.am[1], t.am[2], t.am[3]))) {
+            t_err = "class not built";
+            return SIMINT_ERROR_UNSUPPORTED;
+        }
+        t.nc = ncart(t.am[0]) * ncart(t.am[1]) * ncart(t.am[2]) * ncart(t.am[3]);
+        t.off = scratch;
+        scratch += (size_t)nP * nQ * t.nc;
+    }
     if (nP == 0 || nQ == 0) return 0;
-    const int nc = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
 
     // kets into tiles, grouped by primitive count (stable), so a tile's
     // lanes share one primitive loop
@@ -365,44 +518,77 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
     std::vector<long long> colmap(order.begin(), order.end());
 
     Image im;
-    const LinearOffs lo = pack_linear(im, P, nP);
-    const TiledOffs to = pack_tiled(im, Q, order, colmap);
-    const size_t items_off = im.add(sizeof(WorkItem) * (size_t)nP * to.ntile);
-    WorkItem *items = im.at<WorkItem>(items_off);
-    long long nit = 0;
-    for (int b = 0; b < nP; b++)
-        for (int t = 0; t < to.ntile; t++) {
-            WorkItem &w = items[nit++];
-            w.bra = b;
-            w.tile = t;
-            w.lane_lo = 0;
-            w.lane_hi = std::min(SIMINT_TILE, nQ - t * SIMINT_TILE);
-            w.obase = (long long)b * nQ * nc;
-        }
+    const LinearOffs lo = pack_linear(im, P, nP, deriv > 0);
+    const TiledOffs to = pack_tiled(im, Q, order, colmap, deriv > 0);
+    std::vector<size_t> items_off;
+    const long long nit = (long long)nP * to.ntile;
+    for (auto &t : terms) {
+        const size_t off = im.add(sizeof(WorkItem) * (size_t)nit);
+        items_off.push_back(off);
+        WorkItem *items = im.at<WorkItem>(off);
+        long long k = 0;
+        for (int b = 0; b < nP; b++)
+            for (int tt = 0; tt < to.ntile; tt++) {
+                WorkItem &w = items[k++];
+                w.bra = b;
+                w.tile = tt;
+                w.lane_lo = 0;
+                w.lane_hi = std::min(SIMINT_TILE, nQ - tt * SIMINT_TILE);
+                w.pad = 0;
+                w.obase = (long long)b * nQ * t.nc;
+            }
+    }
 
     ThreadCtx &c = ctx();
     hipError_t e;
+    const size_t final_n = (size_t)nP * nQ * nc * (deriv ? 12 : 1);
+    const size_t out_bytes = sizeof(double) * (scratch + (deriv ? final_n : 0));
     if ((e = c.stage.reserve(im.buf.size())) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
     if ((e = c.in.reserve(im.buf.size())) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
-    const size_t out_bytes = sizeof(double) * (size_t)nP * nQ * nc;
     if ((e = c.out.reserve(out_bytes)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
     memcpy(c.stage.p, im.buf.data(), im.buf.size());
     e = hipMemcpyAsync(c.in.p, c.stage.p, im.buf.size(), hipMemcpyHostToDevice, c.s);
     if (e != hipSuccess) { set_err("hipMemcpyAsync(H2D)", e); return SIMINT_ERROR_HIP; }
 
     char *base = static_cast<char *>(c.in.p);
-    EriArgs a;
-    a.bra = linear_ptrs(base, lo);
-    a.ket = tiled_ptrs(base, to);
-    a.items = reinterpret_cast<const WorkItem *>(base + items_off);
-    a.nitems = nit;
-    a.out = static_cast<double *>(c.out.p);
-    a.boys_grid = g_dev.boys;
-    a.tol2 = (screen_tol > 0.0 && P->screen && Q->screen) ? tol2 : 0.0;
-    a.ket_screen_max = Q->screen_max;
-    a.stride = nc;
-    if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
-    e = hipMemcpyAsync(integrals, c.out.p, out_bytes, hipMemcpyDeviceToHost, c.s);
+    double *dout = static_cast<double *>(c.out.p);
+    for (size_t k = 0; k < terms.size(); k++) {
+        const Term &t = terms[k];
+        EriArgs a;
+        a.bra = linear_ptrs(base, lo);
+        a.ket = tiled_ptrs(base, to);
+        if (t.weight && t.centre == 0) a.bra.prefac = reinterpret_cast<const double *>(base + lo.prefac_a2);
+        if (t.weight && t.centre == 1) a.bra.prefac = reinterpret_cast<const double *>(base + lo.prefac_b2);
+        if (t.weight && t.centre == 2) a.ket.prefac = reinterpret_cast<const double *>(base + to.prefac_a2);
+        if (t.weight && t.centre == 3) a.ket.prefac = reinterpret_cast<const double *>(base + to.prefac_b2);
+        a.items = reinterpret_cast<const WorkItem *>(base + items_off[k]);
+        a.nitems = nit;
+        a.out = dout + t.off;
+        a.boys_grid = g_dev.boys;
+        a.tol2 = (screen_tol > 0.0 && P->screen && Q->screen) ? tol2 : 0.0;
+        a.ket_screen_max = Q->screen_max;
+        a.stride = t.nc;
+        if ((e = t.fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    }
+    const double *result = dout;
+    if (deriv == 1) {
+        Deriv1Args d;
+        for (int q = 0; q < 4; q++) {
+            d.am[q] = am[q];
+            d.plus[q] = nullptr;
+            d.minus[q] = nullptr;
+        }
+        for (const Term &t : terms) {
+            if (t.sign > 0) d.plus[t.centre] = dout + t.off;
+            else d.minus[t.centre] = dout + t.off;
+        }
+        d.skip = skip;
+        d.nquartet = (long long)nP * nQ;
+        d.out = dout + scratch;
+        if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
+        result = d.out;
+    }
+    e = hipMemcpyAsync(integrals, result, sizeof(double) * final_n, hipMemcpyDeviceToHost, c.s);
     if (e != hipSuccess) { set_err("hipMemcpyAsync(D2H)", e); return SIMINT_ERROR_HIP; }
     e = hipStreamSynchronize(c.s);
     if (e != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }

@@ -101,6 +101,7 @@ def plan(am):
 
 
 LDS_BUDGET = 56 * 1024
+LDS_MAX = 160 * 1024 - 1024   # gfx950: 160 KB of LDS per CU, usable by one workgroup
 THREADS = 256
 
 
@@ -117,6 +118,8 @@ def batch_size(am):
         if 8 * (ws * B + PB_SETUP * npc * B) <= LDS_BUDGET:
             break
         B //= 2
+    if 8 * (ws * B + PB_SETUP * npc * B) > LDS_MAX:
+        raise ValueError("class %s needs %d bytes of LDS" % (am, 8 * (ws + PB_SETUP * npc)))
     return B, ws, npc
 
 

@@ -69,3 +69,47 @@ def parity_report(got, ref, offs, sizes, exact=None):
              "median_normwise": float(np.median(nw)) if nw.size else 0.0,
              "worst_vs_exact_bound": worst_ratio}
     return elem_ok and quart_ok, stats
+
+
+DERIV_ABS = 1e-10   # the absolute ERI error bound the engine is held to
+
+
+def parity_report_deriv(got, ref, offs, sizes, exact):
+    """Compare first-derivative integrals with the oracle, given the exact
+    (long-double analytic) values.
+
+    A derivative is 2a (a+1) - a_x (a-1), summed over primitives: when a
+    tight exponent (2a ~ 1e4) meets a small result the two terms cancel, and
+    any FP64 evaluation of the reference algorithm -- the oracle included --
+    is off from the exact value by far more than 1e-14 relative (up to
+    ~1e-10 normwise on a quartet of the 64-atom system).  Two FP64
+    evaluations with a different rounding order differ by the same amount,
+    so an elementwise 1e-14 bar cannot hold.  Pass criteria instead:
+      * every element |got - ref| <= 1e-10 (absolute);
+      * over the batch, the GPU's worst error against the exact values,
+        |got - exact| / max(1, |exact|), is at most max(1e-14, 4 x) the
+        oracle's worst;
+      * the median over the batch of the per-quartet worst errors is at most
+        4x the oracle's (+1e-16): the two are statistically alike (the GPU's
+        median is 1-2x the oracle's on the 64-atom shells).
+    Returns (ok, stats)."""
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    ex = np.asarray(exact)
+    absd = np.abs(got - ref)
+    abs_ok = bool(np.all(absd <= DERIV_ABS))
+    w = np.maximum(1.0, np.abs(ex))
+    eg = np.abs(got - ex) / w
+    eo = np.abs(ref - ex) / w
+    worst_ok = bool(eg.max() <= max(NORMWISE, 4.0 * eo.max())) if eg.size else True
+    qg = np.array([eg[o:o + n].max() for o, n in zip(offs, sizes) if n])
+    qo = np.array([eo[o:o + n].max() for o, n in zip(offs, sizes) if n])
+    med_ok = bool(np.median(qg) <= 4.0 * np.median(qo) + 1e-16) if qg.size else True
+    stats = {"max_abs_vs_oracle": float(absd.max()) if absd.size else 0.0,
+             "gpu_worst_vs_exact": float(eg.max()) if eg.size else 0.0,
+             "oracle_worst_vs_exact": float(eo.max()) if eo.size else 0.0,
+             "gpu_median_vs_exact": float(np.median(qg)) if qg.size else 0.0,
+             "oracle_median_vs_exact": float(np.median(qo)) if qo.size else 0.0,
+             "elementwise_1e-14_fraction_ok": float(np.mean(absd <= ATOL * np.maximum(1.0, np.abs(ref))))
+             if absd.size else 1.0}
+    return abs_ok and worst_ok and med_ok, stats

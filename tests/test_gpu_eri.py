@@ -19,7 +19,7 @@ import pytest
 import oracle
 import simint
 import workloads
-from helpers import mol_shells, normalized_arrays, per_quartet_errors, parity_report
+from helpers import mol_shells, normalized_arrays, parity_report, parity_report_deriv
 
 pytestmark = pytest.mark.gpu
 
@@ -33,24 +33,29 @@ def _oracle_for_dropin(sa, A, B, C, D, screen_method=0, screen_tol=0.0, deriv=0)
     return out, offs, sizes
 
 
-def _check(sa, basis, A, B, C, D, screen=0, tol=0.0, exact=False):
-    """Run simint_compute_eri on P = A x B, Q = C x D and compare with the
-    oracle; returns the parity stats (asserting the bar)."""
+def _check(sa, basis, A, B, C, D, screen=0, tol=0.0, exact=False, deriv=0):
+    """Run simint_compute_eri (or _deriv) on P = A x B, Q = C x D and compare
+    with the oracle; returns the parity stats (asserting the bar)."""
     P = simint.Pairs(basis, A, B, screen)
     Q = simint.Pairs(basis, C, D, screen)
     try:
-        rc, got = simint.compute_eri(P, Q, tol)
+        rc, got = simint.compute_eri(P, Q, tol, deriv)
     finally:
         P.free()
         Q.free()
-    ref, offs, sizes = _oracle_for_dropin(sa, A, B, C, D, screen, tol)
+    ref, offs, sizes = _oracle_for_dropin(sa, A, B, C, D, screen, tol, deriv)
     if rc == -1:
         got = np.zeros_like(ref)
     else:
         assert rc == len(A) * len(B) * len(C) * len(D), (rc, simint.last_error())
+    quartets = [(a, b, c, d) for a in A for b in B for c in C for d in D]
+    if deriv:
+        ex, _, _ = oracle.md_eri_deriv1_analytic(sa, quartets)
+        ok, st = parity_report_deriv(got, ref, offs, sizes, ex)
+        assert ok, ((A, B, C, D), st)
+        return st
     ex = None
     if exact:
-        quartets = [(a, b, c, d) for a in A for b in B for c in C for d in D]
         ex, _, _ = oracle.md_eri(sa, quartets, reference_boys=True)
     ok, st = parity_report(got, ref, offs, sizes, ex)
     assert ok, ((A, B, C, D), st)
@@ -94,6 +99,98 @@ def test_all_classes_molecule(gpu, mol):
         st = _check(sa, basis, byam[i], byam[j], byam[k], byam[l], exact=True)
         worst[(i, j, k, l)] = st["max_normwise"]
     print("worst normwise over classes", max(worst.values()))
+
+
+def _f_shells(seed=11):
+    """Two shells of each AM 0..3 on random centres in a 4-bohr cube,
+    contraction depth 1-3, exponents log-uniform in [0.1, 50]."""
+    rng = np.random.default_rng(seed)
+    shells = []
+    for l in range(4):
+        for _ in range(2):
+            k = int(rng.integers(1, 4))
+            shells.append((l, tuple(rng.uniform(0, 4, 3)), list(np.exp(rng.uniform(np.log(0.1), np.log(50), k))),
+                           list(rng.uniform(0.1, 1.0, k))))
+    return shells
+
+
+def test_f_classes(gpu):
+    """Every class with an f shell: four-centre quartets of random shells,
+    and water aug-cc-pVTZ (contracted 8-primitive s, f on oxygen, d on
+    hydrogen) with the first and last shell of each AM."""
+    for shells, pick in [(_f_shells(), None), (mol_shells("water.aug-cc-pvtz"), "ends")]:
+        basis = simint.Basis(shells)
+        sa = normalized_arrays(shells)
+        byam = {}
+        for n, s in enumerate(shells):
+            byam.setdefault(s[0], []).append(n)
+        if pick == "ends":
+            byam = {l: sorted({v[0], v[-1]}) for l, v in byam.items()}
+        worst = 0.0
+        for am in itertools.product(range(4), repeat=4):
+            if 3 not in am:
+                continue
+            st = _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]], exact=True)
+            worst = max(worst, st["max_normwise"])
+        print("f classes worst normwise", worst)
+
+
+@pytest.mark.parametrize("mol", ["water.6-31gss"])
+def test_deriv1_all_classes(gpu, mol):
+    """First derivatives (simint_compute_eri_deriv, 12 values per Cartesian
+    [Ax Ay Az Bx ... Dz], ostei_deriv1_generator.cpp) for every class up to
+    d against the oracle's deriv = 1 path."""
+    shells = mol_shells(mol)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    ams = sorted(a for a in byam if a <= 2)
+    worst = 0.0
+    for i, j, k, l in itertools.product(ams, repeat=4):
+        st = _check(sa, basis, byam[i], byam[j], byam[k], byam[l], deriv=1)
+        worst = max(worst, st["max_abs_vs_oracle"])
+    print("deriv1 worst |gpu - oracle| over classes", worst)
+
+
+def test_deriv1_tight_exponents(gpu):
+    """The 64-atom recipe's shells (core exponents up to 5000) on two atoms,
+    all classes up to d: the ill-conditioned derivatives."""
+    shells = workloads.config2(natom=2, seed=5)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    for am in itertools.product(range(3), repeat=4):
+        _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]], deriv=1)
+
+
+@pytest.mark.parametrize("screen", [simint.SCREEN_FASTSCHWARZ, simint.SCREEN_SCHWARZ])
+def test_deriv1_screened_and_limits(gpu, screen):
+    """Primitive screening on the derivative path, the same-shell pair
+    weighting (alpha2 = beta2 = a_i + a_j on a triangle pair), and a class
+    above the derivative maximum rejected with SIMINT_ERROR_UNSUPPORTED."""
+    shells = workloads.config2(natom=2, seed=5)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    for am in [(0, 0, 0, 0), (1, 0, 1, 0), (1, 1, 1, 1), (2, 1, 0, 1), (2, 2, 2, 1)]:
+        _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]], screen, 1e-7, deriv=1)
+    # same shell on both sides of a pair
+    s0 = byam[1][0]
+    _check(sa, basis, [s0], [s0], [s0], [s0], deriv=1)
+    # an f shell needs the (g..) classes for its derivative: above the
+    # compiled maximum AM of 3
+    d = byam[2][0]
+    b2 = simint.Basis([(3, shells[d][1], shells[d][2], shells[d][3])])
+    P = simint.Pairs(b2, [0], [0])
+    rc, _ = simint.compute_eri(P, P, 0.0, 1)
+    P.free()
+    assert rc == simint.ERROR_UNSUPPORTED, rc
 
 
 def test_lane_and_wave_kernels_agree(gpu):

@@ -222,3 +222,26 @@ def test_deriv1_against_finite_differences():
         # finite differences carry ~1e-9 relative error; components that
         # vanish by symmetry are compared absolutely
         assert np.abs(a - b).max() <= 1e-8 * scale + 1e-14, (qs[k], np.abs(a - b).max(), scale)
+
+
+def test_md_analytic_deriv1_against_finite_differences():
+    """The long-double analytic first derivatives (the exact values the GPU
+    derivative tests use) against Richardson finite differences of the
+    long-double integrals, and translational invariance."""
+    shells = [
+        (1, (0.1, -0.3, 0.2), [1.3, 0.4], [0.6, 0.5]),
+        (2, (1.0, 0.7, -0.4), [0.9], [1.0]),
+        (0, (-0.6, 0.2, 1.1), [2.1, 0.3], [0.4, 0.7]),
+        (1, (0.3, 1.2, 0.5), [0.7], [1.0]),
+    ]
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    qs = [(0, 1, 2, 3), (1, 1, 3, 2), (3, 0, 0, 2), (2, 2, 2, 2), (0, 0, 0, 0)]
+    ex, offs, sizes = oracle.md_eri_deriv1_analytic(sa, qs)
+    fd, _, _ = oracle.md_eri_deriv1(sa, qs, h=1e-3)
+    assert np.abs(ex - fd).max() <= 1e-9 * max(1.0, np.abs(ex).max())
+    inv = ex.reshape(-1, 4, 3).sum(axis=1)
+    assert np.abs(inv).max() <= 1e-15 * max(1.0, np.abs(ex).max())
+    # the oracle (deriv = 1) is the same quantity
+    ref, _, _ = oracle.eri(sa, qs, deriv=1)
+    assert np.abs(ref - ex).max() <= 1e-13 * max(1.0, np.abs(ex).max())
