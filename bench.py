@@ -74,6 +74,72 @@ def cpu_baseline(ss, seconds=12.0):
                       % (done, t_used)}
 
 
+def batched_leg(steps, warmup, nquartet=10 ** 6):
+    """The batched stream: 1e6 quartets over every class up to f (pool of
+    4096 shells, depths 1-6), one graph launch per pass of 256 quartet-list
+    class launches, inputs and the 5 GB output resident in HBM."""
+    import torch
+    import batch
+    import simint
+    import workloads
+    info = json.load(open(os.path.join(PKG, "csrc", "gen", "classes.json")))
+    t0 = time.perf_counter()
+    shells, q = workloads.config3(nquartet=nquartet, seed=0)
+    basis = simint.Basis(shells)
+    b = batch.QuartetBatch(basis, q)
+    setup = time.perf_counter() - t0
+    flops = 0.0
+    for g in b.groups:
+        c = info["_".join("spdfghik"[x] for x in g.cls)]
+        flops += g.prim_quartets * (c["f_setup"] + c["f_vrr"] + c["f_boys"] + c["bcont"]) + g.n * c["f_hrr"]
+    out = torch.empty(b.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
+    # one eager pass with per-class events (class breakdown)
+    ev = []
+
+    def rec(g, start):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        ev.append((g, e))
+    b.run(out.data_ptr(), events=rec)
+    simint.synchronize()
+    per = []
+    for k in range(0, len(ev), 2):
+        g, e0 = ev[k]
+        _, e1 = ev[k + 1]
+        per.append((e0.elapsed_time(e1), g))
+    gr = b.capture(out.data_ptr())
+    for _ in range(warmup):
+        gr.launch()
+    simint.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t1 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        gr.launch()
+    e1.record(stream)
+    simint.synchronize()
+    wall = time.perf_counter() - t1
+    dev_s = e0.elapsed_time(e1) / 1000.0
+    gr.free()
+    per.sort(key=lambda x: -x[0])
+    res = {"workload": "%d quartets of 4 independent uniform draws from a pool of 4096 shells (1024 per AM "
+                       "0..3, depth U{1..6}, exponents log-uniform [0.05, 1e4], 30-bohr cube), all 256 classes, "
+                       "FP64" % nquartet,
+           "value": nquartet * steps / wall, "unit": "shell quartets/s",
+           "ms_per_pass": 1000.0 * wall / steps, "device_ms_per_pass": 1000.0 * dev_s / steps,
+           "tflops": flops * steps / dev_s / 1e12, "frac_fp64_peak": flops * steps / dev_s / PEAK_FP64,
+           "flops_per_pass": flops, "output_bytes": 8.0 * b.out_doubles,
+           "launch": "one hipGraph per pass holding %d quartet-list class launches" % len(b.groups),
+           "setup_s": setup,
+           "top_classes": [{"class": "_".join("spdfghik"[x] for x in g.cls), "ms": ms, "quartets": g.n,
+                            "kernel": ("lpq" if simint.lib().simint_hip_class_kernel(*g.cls) == 1 else "wpq")}
+                           for ms, g in per[:6]]}
+    b.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +147,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--natom", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched all-classes stream")
     ap.add_argument("--only-class", default=None,
                     help="profiling aid: run only this class's launches (e.g. d_p_d_p)")
     ap.add_argument("--force-family", choices=["lpq", "wpq"], default=None,
@@ -213,6 +280,9 @@ def main():
         "roofline": roof,
         "whole_job": whole,
     }
+    if not args.no_batched and not args.only_class:
+        ss.free()
+        line["batched"] = batched_leg(args.steps, args.warmup)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ss)
     top = sorted(per.items(), key=lambda kv: -kv[1]["ms"])[:8]
@@ -221,7 +291,8 @@ def main():
                             "quartets_per_step": v["quartets"] // args.steps,
                             "kernel": ss.info[quartets.simint_class_name(k)]["kernel"]} for k, v in top]
     print(json.dumps(line))
-    ss.free()
+    if args.no_batched or args.only_class:
+        ss.free()
 
 
 if __name__ == "__main__":

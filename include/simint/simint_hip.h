@@ -33,6 +33,36 @@ long long simint_hip_compute_eri_prefix(simint_hip_pairs const * P, simint_hip_p
                                         long long const * out_off,
                                         double screen_tol, double * d_out);
 
+/* Quartet lists: quartet i is bra pair d_qbra[i] of P against ket pair
+ * d_qket[i] of Q, its ncart(abcd) integrals written at d_out + d_qout[i]
+ * (or at d_out + i * ncart(abcd) when d_qout is NULL).  All arrays are
+ * device arrays, and the call only queues kernels (no host copies or
+ * synchronisation), so it may be captured into a graph.  Quartets of any
+ * mix of primitive counts; ordering the list by primitive count keeps the
+ * lanes of a wavefront in step.  Returns nquartet or a negative SIMINT_ERROR_*. */
+long long simint_hip_compute_eri_list(simint_hip_pairs const * P, simint_hip_pairs const * Q,
+                                      long long nquartet, int const * d_qbra, int const * d_qket,
+                                      long long const * d_qout, double screen_tol, double * d_out);
+
+/* First derivatives of a quartet list (the layout of simint_compute_eri_deriv:
+ * 12 doubles per Cartesian, [Ax Ay Az Bx ... Dz]) at d_out + d_qout[i] (or
+ * i * 12 ncart(abcd)).  d_work: device scratch of simint_hip_deriv1_worksize
+ * doubles.  The class needs its (a+1) classes built (AM <= SIMINT_OSTEI_DERIV1_MAXAM). */
+long long simint_hip_deriv1_worksize(simint_hip_pairs const * P, simint_hip_pairs const * Q, long long nquartet);
+long long simint_hip_compute_eri_deriv1_list(simint_hip_pairs const * P, simint_hip_pairs const * Q,
+                                             long long nquartet, int const * d_qbra, int const * d_qket,
+                                             long long const * d_qout, double screen_tol, double * d_work,
+                                             double * d_out);
+
+/* Graph capture of the calling thread's library stream: every list call
+ * between begin and end is recorded (not run); simint_hip_graph_launch
+ * replays the whole batch as one graph launch. */
+typedef struct simint_hip_graph simint_hip_graph;
+int  simint_hip_capture_begin(void);
+int  simint_hip_capture_end(simint_hip_graph ** out);
+int  simint_hip_graph_launch(simint_hip_graph * g);
+void simint_hip_graph_free(simint_hip_graph * g);
+
 /* (bb|bb) for every pair b of P, at d_out + b * ncart(abab). */
 long long simint_hip_compute_eri_diag(simint_hip_pairs const * P, double * d_out);
 

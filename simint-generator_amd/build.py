@@ -75,9 +75,22 @@ def _compile(src, hdr_hash, want_report):
     return obj, p.stderr, True
 
 
-def _scratch(report):
-    m = re.findall(r"ScratchSize \[bytes/lane\]: (\d+)", report)
-    return max(int(x) for x in m) if m else 0
+def _scratch(report, kernel=None):
+    """Scratch bytes per lane from the compiler's resource report: of the
+    named kernel (mangled name contains it), or the largest in the file."""
+    if kernel is None:
+        m = re.findall(r"ScratchSize \[bytes/lane\]: (\d+)", report)
+        return max(int(x) for x in m) if m else 0
+    cur = None
+    for line in report.splitlines():
+        fm = re.search(r"Function Name: (\S+)", line)
+        if fm:
+            cur = fm.group(1)
+            continue
+        sm = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if sm and cur and re.search(r"\d%s\d" % re.escape(kernel), cur):
+            return int(sm.group(1))
+    return 0
 
 
 def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
@@ -113,7 +126,7 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
     ok = set()
     for am in lpq:
         s = os.path.join(GEN, "eri_lpq_%s.hip" % class_name(am))
-        if _scratch(reports[s]) == 0:
+        if _scratch(reports[s], "eri_lpq_" + class_name(am)) == 0:
             ok.add(am)
     with open(okf, "w") as f:
         json.dump({"maxam": maxam, "ok": sorted(list(a) for a in ok)}, f)

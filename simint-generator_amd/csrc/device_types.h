@@ -56,6 +56,24 @@ struct WorkItem {
     long long obase;
 };
 
+// Quartet-list work (any bra against any ket): quartet q of a launch is
+// bra pair qbra[q] against ket pair qket[q], both in the linear view, its
+// ncart(abcd) integrals written contiguously at out + qout[q] (or, with
+// qout null, at out + q * ostride).
+struct EriListArgs {
+    PairsLinear bra;
+    PairsLinear ket;
+    const int *qbra;
+    const int *qket;
+    const long long *qout;
+    long long ostride;
+    long long nquartet;
+    double *out;
+    const double *boys_grid;
+    double tol2;
+    double ket_screen_max;
+};
+
 struct EriArgs {
     PairsLinear bra;
     PairsTiled ket;

@@ -7,4 +7,9 @@
 
 typedef hipError_t (*simint_launch_fn)(const EriArgs *, hipStream_t);
 
+typedef hipError_t (*simint_list_launch_fn)(const EriListArgs *, hipStream_t);
+
 simint_launch_fn simint_dispatch(int la, int lb, int lc, int ld);
+// quartet-list launcher of the class's selected family (or the other family
+// when the selected one has no list kernel); null if the class is not built
+simint_list_launch_fn simint_dispatch_list(int la, int lb, int lc, int ld);

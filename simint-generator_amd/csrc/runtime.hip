@@ -351,11 +351,12 @@ int validate(const simint_multi_shellpair *P)
 // [Ax Ay Az Bx ... Dz].
 struct Deriv1Args {
     int am[4];
-    const double *plus[4];
-    const double *minus[4];
+    const double *plus[4];     // per explicit centre: (a+1) class, contiguous per quartet
+    const double *minus[4];    // (a-1) class (null where a = 0)
     int skip;
     long long nquartet;
     double *out;
+    const long long *qout;     // per quartet output offset, or null: contiguous
 };
 
 __device__ inline int dev_ncart(int l) { return (l + 1) * (l + 2) / 2; }
@@ -372,7 +373,8 @@ __global__ void deriv1_assemble(const Deriv1Args d)
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= d.nquartet * nk) return;
     const long long q = gid / nk;
-    int k = (int)(gid - q * nk);
+    const int k0 = (int)(gid - q * nk);
+    int k = k0;
     int comp[4];
     for (int c = 3; c >= 0; c--) {
         comp[c] = k % n[c];
@@ -418,7 +420,7 @@ __global__ void deriv1_assemble(const Deriv1Args d)
             if (c != d.skip) s -= res[3 * c + x];
         res[3 * d.skip + x] = s;
     }
-    double *o = d.out + gid * 12;
+    double *o = d.qout ? d.out + d.qout[q] + (long long)k0 * 12 : d.out + gid * 12;
     for (int t = 0; t < 12; t++) o[t] = res[t];
 }
 
@@ -585,6 +587,7 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
         d.skip = skip;
         d.nquartet = (long long)nP * nQ;
         d.out = dout + scratch;
+        d.qout = nullptr;
         if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
         result = d.out;
     }
@@ -619,6 +622,7 @@ struct simint_hip_pairs {
     PairsLinear lin;
     PairsTiled til;
     int ntile;
+    const double *wa = nullptr, *wb = nullptr;   // linear prefac * alpha2, prefac * beta2
 };
 
 int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **out)
@@ -628,7 +632,7 @@ int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **
     if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
     const int n = P->nshell12_clip;
     Image im;
-    const LinearOffs lo = pack_linear(im, P, n);
+    const LinearOffs lo = pack_linear(im, P, n, true);
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::vector<long long> colmap(order.begin(), order.end());
@@ -648,6 +652,8 @@ int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **
     p->lin = linear_ptrs(static_cast<char *>(d), lo);
     p->til = tiled_ptrs(static_cast<char *>(d), to);
     p->ntile = to.ntile;
+    p->wa = reinterpret_cast<const double *>(static_cast<char *>(d) + lo.prefac_a2);
+    p->wb = reinterpret_cast<const double *>(static_cast<char *>(d) + lo.prefac_b2);
     *out = p;
     return 0;
 }
@@ -784,6 +790,188 @@ long long simint_hip_compute_eri_diag(const simint_hip_pairs *P, double *d_out)
     a.stride = ncart(P->am1) * ncart(P->am2) * ncart(P->am1) * ncart(P->am2);
     if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
     return P->n;
+}
+
+// ------------------------------------------------------------ quartet lists
+namespace {
+
+struct DerivPlan {
+    int am[4];
+    int skip;
+    struct T { int am[4]; int centre, sign, weight; long long nc; size_t off; } t[8];
+    int nt = 0;
+    size_t scratch = 0;   // doubles per quartet
+};
+
+DerivPlan deriv1_plan(const int am[4])
+{
+    DerivPlan d;
+    for (int c = 0; c < 4; c++) d.am[c] = am[c];
+    const int mx = std::max(std::max(am[0], am[1]), std::max(am[2], am[3]));
+    d.skip = 0;
+    for (int c = 0; c < 4; c++)
+        if (am[c] == mx) { d.skip = c; break; }
+    for (int c = 0; c < 4; c++) {
+        if (c == d.skip) continue;
+        for (int sgn = +1; sgn >= -1; sgn -= 2) {
+            if (sgn < 0 && am[c] == 0) continue;
+            DerivPlan::T t;
+            for (int k = 0; k < 4; k++) t.am[k] = am[k];
+            t.am[c] += sgn;
+            t.centre = c;
+            t.sign = sgn;
+            t.weight = sgn > 0;
+            t.nc = (long long)ncart(t.am[0]) * ncart(t.am[1]) * ncart(t.am[2]) * ncart(t.am[3]);
+            t.off = d.scratch;
+            d.scratch += (size_t)t.nc;
+            d.t[d.nt++] = t;
+        }
+    }
+    return d;
+}
+
+EriListArgs list_args(const simint_hip_pairs *P, const simint_hip_pairs *Q, long long n, const int *qbra,
+                      const int *qket, double screen_tol)
+{
+    EriListArgs a;
+    a.bra = P->lin;
+    a.ket = Q->lin;
+    a.qbra = qbra;
+    a.qket = qket;
+    a.qout = nullptr;
+    a.ostride = 0;
+    a.nquartet = n;
+    a.out = nullptr;
+    a.boys_grid = g_dev.boys;
+    a.tol2 = (screen_tol > 0.0 && P->has_screen && Q->has_screen) ? screen_tol * screen_tol : 0.0;
+    a.ket_screen_max = Q->screen_max;
+    return a;
+}
+
+}  // namespace
+
+long long simint_hip_compute_eri_list(const simint_hip_pairs *P, const simint_hip_pairs *Q, long long nquartet,
+                                      const int *d_qbra, const int *d_qket, const long long *d_qout,
+                                      double screen_tol, double *d_out)
+{
+    if (!P || !Q || nquartet < 0 || (nquartet > 0 && (!d_qbra || !d_qket || !d_out))) return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const simint_list_launch_fn fn = simint_dispatch_list(P->am1, P->am2, Q->am1, Q->am2);
+    if (!fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    if (nquartet == 0) return 0;
+    EriListArgs a = list_args(P, Q, nquartet, d_qbra, d_qket, screen_tol);
+    a.qout = d_qout;
+    a.ostride = (long long)ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
+    a.out = d_out;
+    hipError_t e = fn(&a, ctx().s);
+    if (e != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    return nquartet;
+}
+
+long long simint_hip_deriv1_worksize(const simint_hip_pairs *P, const simint_hip_pairs *Q, long long nquartet)
+{
+    if (!P || !Q || nquartet < 0) return SIMINT_ERROR_ARGUMENT;
+    const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
+    return (long long)deriv1_plan(am).scratch * nquartet;
+}
+
+long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const simint_hip_pairs *Q,
+                                             long long nquartet, const int *d_qbra, const int *d_qket,
+                                             const long long *d_qout, double screen_tol, double *d_work,
+                                             double *d_out)
+{
+    if (!P || !Q || nquartet < 0 || (nquartet > 0 && (!d_qbra || !d_qket || !d_out || !d_work)))
+        return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
+    const DerivPlan dp = deriv1_plan(am);
+    simint_list_launch_fn fns[8];
+    for (int k = 0; k < dp.nt; k++) {
+        const DerivPlan::T &t = dp.t[k];
+        fns[k] = nullptr;
+        if (t.am[0] <= SIMINT_OSTEI_MAXAM && t.am[1] <= SIMINT_OSTEI_MAXAM && t.am[2] <= SIMINT_OSTEI_MAXAM &&
+            t.am[3] <= SIMINT_OSTEI_MAXAM)
+            fns[k] = simint_dispatch_list(t.am[0], t.am[1], t.am[2], t.am[3]);
+        if (!fns[k]) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    }
+    if (nquartet == 0) return 0;
+    ThreadCtx &c = ctx();
+    hipError_t e;
+    Deriv1Args d;
+    for (int q = 0; q < 4; q++) {
+        d.am[q] = am[q];
+        d.plus[q] = nullptr;
+        d.minus[q] = nullptr;
+    }
+    for (int k = 0; k < dp.nt; k++) {
+        const DerivPlan::T &t = dp.t[k];
+        EriListArgs a = list_args(P, Q, nquartet, d_qbra, d_qket, screen_tol);
+        if (t.weight) {
+            if (t.centre == 0) a.bra.prefac = P->wa;
+            if (t.centre == 1) a.bra.prefac = P->wb;
+            if (t.centre == 2) a.ket.prefac = Q->wa;
+            if (t.centre == 3) a.ket.prefac = Q->wb;
+        }
+        a.out = d_work + t.off * nquartet;
+        a.ostride = t.nc;
+        if ((e = fns[k](&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+        if (t.sign > 0) d.plus[t.centre] = a.out;
+        else d.minus[t.centre] = a.out;
+    }
+    d.skip = dp.skip;
+    d.nquartet = nquartet;
+    d.out = d_out;
+    d.qout = d_qout;
+    if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
+    return nquartet;
+}
+
+// ------------------------------------------------------------ graph capture
+struct simint_hip_graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+};
+
+int simint_hip_capture_begin(void)
+{
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    hipError_t e = hipStreamBeginCapture(ctx().s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) { set_err("hipStreamBeginCapture", e); return SIMINT_ERROR_HIP; }
+    return 0;
+}
+
+int simint_hip_capture_end(simint_hip_graph **out)
+{
+    if (!out) return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    simint_hip_graph *g = new simint_hip_graph;
+    hipError_t e = hipStreamEndCapture(ctx().s, &g->g);
+    if (e != hipSuccess) { delete g; set_err("hipStreamEndCapture", e); return SIMINT_ERROR_HIP; }
+    e = hipGraphInstantiate(&g->x, g->g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g->g);
+        delete g;
+        set_err("hipGraphInstantiate", e);
+        return SIMINT_ERROR_HIP;
+    }
+    *out = g;
+    return 0;
+}
+
+int simint_hip_graph_launch(simint_hip_graph *g)
+{
+    if (!g) return SIMINT_ERROR_ARGUMENT;
+    hipError_t e = hipGraphLaunch(g->x, ctx().s);
+    if (e != hipSuccess) { set_err("hipGraphLaunch", e); return SIMINT_ERROR_HIP; }
+    return 0;
+}
+
+void simint_hip_graph_free(simint_hip_graph *g)
+{
+    if (!g) return;
+    if (g->x) hipGraphExecDestroy(g->x);
+    if (g->g) hipGraphDestroy(g->g);
+    delete g;
 }
 
 int simint_hip_synchronize(void)

@@ -257,4 +257,64 @@ def emit_lpq(am):
     w("    hipLaunchKernelGGL(eri_lpq_%s, dim3((unsigned)a->nitems), dim3(64), 0, s, *a);" % name)
     w("    return hipGetLastError();")
     w("}")
+    w("")
+    emit_lpq_list(am, w)
     return w.text()
+
+
+def emit_lpq_list(am, w):
+    """Quartet-list variant: lane l of block k takes quartet 64k + l, with
+    its own bra and ket pair (both read from the linear view, per-lane
+    gathers), its own primitive loops, and a contiguous output block."""
+    g = graph(*am)
+    name = class_name(am)
+    ntop = len(g.top_keys)
+    w("__global__ void __launch_bounds__(64) eri_lpql_%s(const EriListArgs A)" % name)
+    w("{")
+    w("    const long long q = (long long)blockIdx.x * 64 + threadIdx.x;")
+    w("    if (q >= A.nquartet) return;")
+    w("    const int bp = A.qbra[q], kp = A.qket[q];")
+    w("    const double *__restrict__ grid = A.boys_grid;")
+    w("    const int ib0 = A.bra.poff[bp];")
+    w("    const int ib1 = ib0 + A.bra.np[bp];")
+    w("    const int jb0 = A.ket.poff[kp];")
+    w("    const int jb1 = jb0 + A.ket.np[kp];")
+    w("    const bool screen = A.tol2 > 0.0;")
+    for t in range(ntop):
+        w("    double acc%d = 0.0;" % t)
+    bra_pa = "pa" if g.bra_j else "pb"
+    ket_pa = "pa" if g.ket_l else "pb"
+    w("    for (int i = ib0; i < ib1; ++i) {")
+    w("        const double P_alpha = A.bra.alpha[i];")
+    w("        const double P_inv = A.bra.inv_alpha[i];")
+    w("        const double P_prefac = A.bra.prefac[i];")
+    w("        const double P_x = A.bra.x[i], P_y = A.bra.y[i], P_z = A.bra.z[i];")
+    w("        const double P_%s[3] = {A.bra.%sx[i], A.bra.%sy[i], A.bra.%sz[i]};"
+      % ("PA" if g.bra_j else "PB", bra_pa, bra_pa, bra_pa))
+    w("        const double P_screen = screen ? A.bra.screen[i] : 0.0;")
+    w("        if (screen && P_screen * A.ket_screen_max < A.tol2) continue;")
+    w("        for (int kj = jb0; kj < jb1; ++kj) {")
+    w("            if (screen && A.ket.screen[kj] * P_screen < A.tol2) continue;")
+    w("            const double Q_alpha = A.ket.alpha[kj];")
+    w("            const double Q_inv = A.ket.inv_alpha[kj];")
+    w("            const double Q_prefac = A.ket.prefac[kj];")
+    w("            const double Q_x = A.ket.x[kj], Q_y = A.ket.y[kj], Q_z = A.ket.z[kj];")
+    w("            const double Q_%s[3] = {A.ket.%sx[kj], A.ket.%sy[kj], A.ket.%sz[kj]};"
+      % ("PA" if g.ket_l else "PB", ket_pa, ket_pa, ket_pa))
+    emit_body(am, w, indent="            ")
+    w("        }")
+    w("    }")
+    w("    double *__restrict__ o = A.out + (A.qout ? A.qout[q] : q * A.ostride);")
+    if g.bra_hrr_needed():
+        w("    const double hAB[3] = {A.bra.abx[bp], A.bra.aby[bp], A.bra.abz[bp]};")
+    if g.ket_hrr_needed():
+        w("    const double hCD[3] = {A.ket.abx[kp], A.ket.aby[kp], A.ket.abz[kp]};")
+    emit_hrr(am, w)
+    w("}")
+    w("")
+    w('extern "C" hipError_t simint_launch_lpql_%s(const EriListArgs *a, hipStream_t s)' % name)
+    w("{")
+    w("    if (a->nquartet <= 0) return hipSuccess;")
+    w("    hipLaunchKernelGGL(eri_lpql_%s, dim3((unsigned)((a->nquartet + 63) / 64)), dim3(64), 0, s, *a);" % name)
+    w("    return hipGetLastError();")
+    w("}")

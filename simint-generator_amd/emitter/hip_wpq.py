@@ -159,31 +159,68 @@ def emit_wpq(am):
     w("__constant__ unsigned short top_slot[%d] = {%s};" % (ntop, ", ".join(map(str, tops))))
     w("}  // namespace")
     w("")
+    _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode=False)
+    _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode=True)
+    return w.text()
+
+
+QCH = 64   # quartets per workgroup in the quartet-list kernels
+
+
+def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
+    """The cooperative kernel: over the ket lanes of a work item (one bra), or
+    (listmode) over a chunk of a quartet list, each quartet with its own bra
+    and ket pair read from the linear view."""
     w("// %d threads cooperate on B = %d quartets at a time; thread t serves quartet" % (THREADS, B))
     w("// slot t %% B and is rank t / B of that quartet's LPB = %d threads." % LPB)
-    w("__global__ void __launch_bounds__(%d) eri_wpq_%s(const EriArgs A)" % (THREADS, name))
+    if listmode:
+        w("__global__ void __launch_bounds__(%d) eri_wpql_%s(const EriListArgs A)" % (THREADS, name))
+    else:
+        w("__global__ void __launch_bounds__(%d) eri_wpq_%s(const EriArgs A)" % (THREADS, name))
     w("{")
     w("    __shared__ double work[WS * B];")
     w("    __shared__ double pc[PB * NPC * B];")
-    w("    const WorkItem wi = A.items[blockIdx.x];")
+    bra_pa = "pa" if g.bra_j else "pb"
+    ket_pa = "pa" if g.ket_l else "pb"
+    if not listmode:
+        w("    const WorkItem wi = A.items[blockIdx.x];")
     w("    const int tid = threadIdx.x;")
     w("    const int b = tid % B;")
     w("    const int r0 = tid / B;")
     w("    const double *__restrict__ grid = A.boys_grid;")
-    w("    const int ib0 = A.bra.poff[wi.bra];")
-    w("    const int npb = A.bra.np[wi.bra];")
-    w("    const int nq = A.ket.tnp[wi.tile];")
-    w("    const long long kt = A.ket.toff[wi.tile];")
     w("    const bool screen = A.tol2 > 0.0;")
-    w("    const int K = npb * nq;")
-    if g.bra_hrr_needed():
-        w("    const double hAB[3] = {A.bra.abx[wi.bra], A.bra.aby[wi.bra], A.bra.abz[wi.bra]};")
-    bra_pa = "pa" if g.bra_j else "pb"
-    ket_pa = "pa" if g.ket_l else "pb"
-    w("    for (int q0 = wi.lane_lo; q0 < wi.lane_hi; q0 += B) {")
-    w("        const int nb = (wi.lane_hi - q0) < B ? (wi.lane_hi - q0) : B;")
-    w("        const int ql = (q0 + b) < SIMINT_TILE ? (q0 + b) : (SIMINT_TILE - 1);")
-    w("        const long long tl = (long long)wi.tile * SIMINT_TILE + ql;")
+    if not listmode:
+        w("    const int ib0 = A.bra.poff[wi.bra];")
+        w("    const int npb = A.bra.np[wi.bra];")
+        w("    const int nq = A.ket.tnp[wi.tile];")
+        w("    const long long kt = A.ket.toff[wi.tile];")
+        w("    const int K = npb * nq;")
+        if g.bra_hrr_needed():
+            w("    const double hAB[3] = {A.bra.abx[wi.bra], A.bra.aby[wi.bra], A.bra.abz[wi.bra]};")
+        w("    for (int q0 = wi.lane_lo; q0 < wi.lane_hi; q0 += B) {")
+        w("        const int nb = (wi.lane_hi - q0) < B ? (wi.lane_hi - q0) : B;")
+        w("        const int ql = (q0 + b) < SIMINT_TILE ? (q0 + b) : (SIMINT_TILE - 1);")
+        w("        const long long tl = (long long)wi.tile * SIMINT_TILE + ql;")
+    else:
+        w("    __shared__ int kk[B];")
+        w("    const long long qbeg = (long long)blockIdx.x * %d;" % QCH)
+        w("    const long long qend = (qbeg + %d) < A.nquartet ? (qbeg + %d) : A.nquartet;" % (QCH, QCH))
+        w("    for (long long q0 = qbeg; q0 < qend; q0 += B) {")
+        w("        const int nb = (qend - q0) < B ? (int)(qend - q0) : B;")
+        w("        const long long qi = (q0 + b) < qend ? (q0 + b) : (qend - 1);")
+        w("        const int bp = A.qbra[qi], kp = A.qket[qi];")
+        w("        const int ib0 = A.bra.poff[bp];")
+        w("        const int nq = A.ket.np[kp];")
+        w("        const int jb0 = A.ket.poff[kp];")
+        w("        const int Kb = (b < nb) ? A.bra.np[bp] * nq : 0;")
+        w("        __syncthreads();")
+        w("        if (r0 == 0) kk[b] = Kb;")
+        w("        __syncthreads();")
+        w("        int K = 0;")
+        w("#pragma unroll")
+        w("        for (int t = 0; t < B; ++t) K = kk[t] > K ? kk[t] : K;")
+        if g.bra_hrr_needed():
+            w("        const double hAB[3] = {A.bra.abx[bp], A.bra.aby[bp], A.bra.abz[bp]};")
     w("        double acc[NACC];")
     w("#pragma unroll")
     w("        for (int s = 0; s < NACC; ++s) acc[s] = 0.0;")
@@ -191,12 +228,24 @@ def emit_wpq(am):
     w("            const int np_batch = (K - p0) < PB ? (K - p0) : PB;")
     w("            __syncthreads();")
     w("            if (r0 < np_batch) {")
-    w("                const int p = p0 + r0;")
-    w("                const int i = ib0 + p / nq;")
-    w("                const long long kj = kt + ql + (long long)SIMINT_TILE * (p % nq);")
+    if not listmode:
+        w("                const int p = p0 + r0;")
+        w("                const int i = ib0 + p / nq;")
+        w("                const long long kj = kt + ql + (long long)SIMINT_TILE * (p % nq);")
+    else:
+        # quartets of the batch with fewer primitive quartets than the
+        # largest contribute zero for the surplus rows
+        w("                const int p = p0 + r0;")
+        w("                const bool live = p < Kb;")
+        w("                const int pe = live ? p : 0;")
+        w("                const int i = ib0 + pe / nq;")
+        w("                const long long kj = jb0 + pe % nq;")
     w("                const double P_alpha = A.bra.alpha[i], P_inv = A.bra.inv_alpha[i];")
     w("                const double Q_alpha = A.ket.alpha[kj], Q_inv = A.ket.inv_alpha[kj];")
-    w("                double P_prefac = A.bra.prefac[i];")
+    if not listmode:
+        w("                double P_prefac = A.bra.prefac[i];")
+    else:
+        w("                double P_prefac = live ? A.bra.prefac[i] : 0.0;")
     w("                if (screen && (A.bra.screen[i] * A.ket_screen_max < A.tol2 ||")
     w("                               A.bra.screen[i] * A.ket.screen[kj] < A.tol2)) P_prefac = 0.0;")
     w("                const double Q_prefac = A.ket.prefac[kj];")
@@ -275,21 +324,30 @@ def emit_wpq(am):
     w("            if (t < NTOP) tops[t * B + b] = acc[s];")
     w("        }")
     w("        __syncthreads();")
-    _emit_hrr_coop(g, w, am)
+    _emit_hrr_coop(g, w, am, listmode)
     w("        __syncthreads();")
     w("    }")
     w("}")
     w("")
-    w('extern "C" hipError_t simint_launch_wpq_%s(const EriArgs *a, hipStream_t s)' % name)
-    w("{")
-    w("    if (a->nitems <= 0) return hipSuccess;")
-    w("    hipLaunchKernelGGL(eri_wpq_%s, dim3((unsigned)a->nitems), dim3(%d), 0, s, *a);" % (name, THREADS))
-    w("    return hipGetLastError();")
-    w("}")
-    return w.text()
+    if not listmode:
+        w('extern "C" hipError_t simint_launch_wpq_%s(const EriArgs *a, hipStream_t s)' % name)
+        w("{")
+        w("    if (a->nitems <= 0) return hipSuccess;")
+        w("    hipLaunchKernelGGL(eri_wpq_%s, dim3((unsigned)a->nitems), dim3(%d), 0, s, *a);" % (name, THREADS))
+        w("    return hipGetLastError();")
+        w("}")
+        w("")
+    else:
+        w('extern "C" hipError_t simint_launch_wpql_%s(const EriListArgs *a, hipStream_t s)' % name)
+        w("{")
+        w("    if (a->nquartet <= 0) return hipSuccess;")
+        w("    hipLaunchKernelGGL(eri_wpql_%s, dim3((unsigned)((a->nquartet + %d) / %d)), dim3(%d), 0, s, *a);"
+          % (name, QCH - 1, QCH, THREADS))
+        w("    return hipGetLastError();")
+        w("}")
 
 
-def _emit_hrr_coop(g, w, am):
+def _emit_hrr_coop(g, w, am, listmode=False):
     """HRR for the B quartets of a batch: bra HRR with lanes over (ket-top
     component f, quartet), then ket HRR with lanes over (final bra
     component, quartet).  LDS layout [value][quartet slot]."""
@@ -324,9 +382,14 @@ def _emit_hrr_coop(g, w, am):
     w(ind + "}")
     w(ind + "__syncthreads();")
     w(ind + "if (b < nb) {")
-    if g.ket_hrr_needed():
-        w(ind + "    const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
-    w(ind + "    double *__restrict__ o = A.out + wi.obase + A.ket.col[tl] * A.stride;")
+    if listmode:
+        if g.ket_hrr_needed():
+            w(ind + "    const double hCD[3] = {A.ket.abx[kp], A.ket.aby[kp], A.ket.abz[kp]};")
+        w(ind + "    double *__restrict__ o = A.out + (A.qout ? A.qout[qi] : qi * A.ostride);")
+    else:
+        if g.ket_hrr_needed():
+            w(ind + "    const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
+        w(ind + "    double *__restrict__ o = A.out + wi.obase + A.ket.col[tl] * A.stride;")
     w(ind + "    for (int ab = r0; ab < NAB; ab += LPB) {")
     w(ind + "        const double *R = abf + ab * NF * B + b;")
     w(ind + "        double *oo = o + ab * NCD;")

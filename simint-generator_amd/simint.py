@@ -56,6 +56,8 @@ EXPORTS = [
     "simint_hip_pairs_create", "simint_hip_pairs_free", "simint_hip_pairs_count",
     "simint_hip_compute_eri_prefix", "simint_hip_compute_eri_diag", "simint_hip_synchronize",
     "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
+    "simint_hip_compute_eri_list", "simint_hip_deriv1_worksize", "simint_hip_compute_eri_deriv1_list",
+    "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
 ]
 
 _LIB = None
@@ -94,6 +96,18 @@ def lib():
         L.simint_hip_compute_eri_prefix.restype = ctypes.c_longlong
         L.simint_hip_compute_eri_diag.argtypes = [c_void_p, c_void_p]
         L.simint_hip_compute_eri_diag.restype = ctypes.c_longlong
+        ll = ctypes.c_longlong
+        L.simint_hip_compute_eri_list.argtypes = [c_void_p, c_void_p, ll, c_void_p, c_void_p, c_void_p, c_double,
+                                                  c_void_p]
+        L.simint_hip_compute_eri_list.restype = ll
+        L.simint_hip_deriv1_worksize.argtypes = [c_void_p, c_void_p, ll]
+        L.simint_hip_deriv1_worksize.restype = ll
+        L.simint_hip_compute_eri_deriv1_list.argtypes = [c_void_p, c_void_p, ll, c_void_p, c_void_p, c_void_p,
+                                                         c_double, c_void_p, c_void_p]
+        L.simint_hip_compute_eri_deriv1_list.restype = ll
+        L.simint_hip_capture_end.argtypes = [ctypes.POINTER(c_void_p)]
+        L.simint_hip_graph_launch.argtypes = [c_void_p]
+        L.simint_hip_graph_free.argtypes = [c_void_p]
         L.simint_hip_stream.restype = c_void_p
         L.simint_hip_last_error.restype = ctypes.c_char_p
         L.simint_hip_class_kernel.argtypes = [c_int] * 4
@@ -221,6 +235,70 @@ class DevicePairs:
         if self.h:
             lib().simint_hip_pairs_free(self.h)
             self.h = None
+
+
+def compute_list(P, Q, n, d_qbra, d_qket, d_qout, d_out, screen_tol=0.0):
+    """simint_hip_compute_eri_list on device arrays (pointers as ints;
+    d_qout may be 0 for contiguous output)."""
+    r = lib().simint_hip_compute_eri_list(P.h, Q.h, int(n), c_void_p(d_qbra), c_void_p(d_qket),
+                                          c_void_p(d_qout or None), float(screen_tol), c_void_p(d_out))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_eri_list failed (%d): %s" % (r, last_error()))
+    return r
+
+
+def deriv1_worksize(P, Q, n):
+    return lib().simint_hip_deriv1_worksize(P.h, Q.h, int(n))
+
+
+def compute_deriv1_list(P, Q, n, d_qbra, d_qket, d_qout, d_work, d_out, screen_tol=0.0):
+    r = lib().simint_hip_compute_eri_deriv1_list(P.h, Q.h, int(n), c_void_p(d_qbra), c_void_p(d_qket),
+                                                 c_void_p(d_qout or None), float(screen_tol), c_void_p(d_work),
+                                                 c_void_p(d_out))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_eri_deriv1_list failed (%d): %s" % (r, last_error()))
+    return r
+
+
+class Graph:
+    """A batch of list calls recorded on the library stream and replayed as
+    one graph launch: with Graph.capture() as g: ...calls...; g.launch()."""
+
+    def __init__(self):
+        self.h = None
+
+    @classmethod
+    def capture(cls):
+        return _Capture(cls())
+
+    def launch(self):
+        r = lib().simint_hip_graph_launch(self.h)
+        if r != 0:
+            raise RuntimeError("simint_hip_graph_launch failed (%d): %s" % (r, last_error()))
+
+    def free(self):
+        if self.h:
+            lib().simint_hip_graph_free(self.h)
+            self.h = None
+
+
+class _Capture:
+    def __init__(self, g):
+        self.g = g
+
+    def __enter__(self):
+        r = lib().simint_hip_capture_begin()
+        if r != 0:
+            raise RuntimeError("simint_hip_capture_begin failed (%d): %s" % (r, last_error()))
+        return self.g
+
+    def __exit__(self, et, ev, tb):
+        h = c_void_p()
+        r = lib().simint_hip_capture_end(ctypes.byref(h))
+        if r != 0 and et is None:
+            raise RuntimeError("simint_hip_capture_end failed (%d): %s" % (r, last_error()))
+        self.g.h = h
+        return False
 
 
 def compute_prefix(P, Q, b0, nket, out_off, d_out_ptr, screen_tol=0.0):

@@ -13,6 +13,11 @@ x, y, z (or per atom), then the exponents, then the coefficients.
       shells (exponents log-uniform in [0.1, 5000]) and s(1), p(1), d(1)
       valence shells (log-uniform in [0.05, 2]); coefficients U[0.1, 1].
       natom=64 is a 4x4x4 grid, natom=256 an 8x8x4 grid.
+  config3(nquartet=10**6, seed=0): a pool of 4096 shells, 1024 of each AM
+      0..3 (in that order); per shell the contraction depth U{1..6}, the
+      centre uniform in a 30-bohr cube, exponents log-uniform in
+      [0.05, 1e4], coefficients U[0.1, 1]; then nquartet quartets of four
+      independent uniform pool indices (every class up to f).
 """
 import math
 
@@ -40,6 +45,20 @@ class SplitMix64:
 
     def index(self, n):
         return min(int(self.uniform() * n), n - 1)
+
+    def block(self, n):
+        """The next n outputs as a uint64 array (same values as n next())."""
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.s) + k * np.uint64(0x9E3779B97F4A7C15)
+            self.s = int(z[-1]) if n else self.s
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+    def indices(self, count, n):
+        u = (self.block(count) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return np.minimum((u * n).astype(np.int64), n - 1)
 
 
 def config1(seed=0, nshell=40, nquartet=1000):
@@ -94,6 +113,26 @@ def config2(natom=64, seed=0, spacing=3.0, jitter=0.3):
                     co = [r.uniform(0.1, 1.0) for _ in range(n)]
                     shells.append((am, xyz, al, co))
     return shells
+
+
+def config3_pool(seed=0, per_am=1024, maxam=3, cube=30.0):
+    r = SplitMix64(seed)
+    shells = []
+    for am in range(maxam + 1):
+        for _ in range(per_am):
+            k = 1 + r.index(6)
+            xyz = (r.uniform(0, cube), r.uniform(0, cube), r.uniform(0, cube))
+            al = [r.loguniform(0.05, 1e4) for _ in range(k)]
+            co = [r.uniform(0.1, 1.0) for _ in range(k)]
+            shells.append((am, xyz, al, co))
+    return shells, r
+
+
+def config3(nquartet=10 ** 6, seed=0):
+    """(shells, quartets[nquartet, 4]) of the mixed-class stream."""
+    shells, r = config3_pool(seed)
+    q = r.indices(4 * nquartet, len(shells)).reshape(nquartet, 4)
+    return shells, q
 
 
 def unique_pairs_by_class(shells, maxam=2):

@@ -40,17 +40,26 @@ def parity_report(got, ref, offs, sizes, exact=None):
     """Compare a result with the oracle per shell quartet.
 
     Pass criteria (both must hold):
-      * elementwise |got - ref| <= 1e-14 * max(1, |ref|);
+      * elementwise |got - ref| <= 1e-14 * max(1, |ref|), or, when `exact`
+        (long-double McMurchie-Davidson values) is given, <= 4 * that
+        quartet's max|ref - exact| if larger: where the reference algorithm
+        itself is further than 1e-14 from the exact value (high-AM classes
+        with tight exponents, HRR cancellation) two FP64 evaluations cannot
+        agree to 1e-14 either;
       * per quartet, max|got - ref| <= max(1e-14 * max|ref|, 4 * max|ref - exact| + 1e-300)
-        when `exact` (long-double McMurchie-Davidson values) is given, i.e. the
-        two FP64 results may differ by no more than the oracle's own error;
-        without `exact` the first bound alone applies per quartet as well as
-        elementwise.
+        when `exact` is given, i.e. the two FP64 results may differ by no
+        more than the oracle's own error.
     Returns (ok, stats)."""
     got = np.asarray(got)
     ref = np.asarray(ref)
     d = np.abs(got - ref)
-    elem_ok = bool(np.all(d <= ATOL * np.maximum(1.0, np.abs(ref))))
+    ebar = ATOL * np.maximum(1.0, np.abs(ref))
+    if exact is not None:
+        for o, n in zip(offs, sizes):
+            if n:
+                eq = np.abs(ref[o:o + n] - exact[o:o + n]).max()
+                ebar[o:o + n] = np.maximum(ebar[o:o + n], 4.0 * eq)
+    elem_ok = bool(np.all(d <= ebar))
     nw = per_quartet_errors(got, ref, offs, sizes)
     quart_ok = True
     worst_ratio = 0.0
