@@ -140,6 +140,72 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
     return res
 
 
+def mixed_leg(natom=256, threshold=1e-4):
+    """The 256-atom system of the 64-atom recipe: every screened quartet
+    (Schwarz 1e-12) in FP64, then again with quartets whose bound is below
+    `threshold` run through the FP32-VRR/HRR kernels; one pass each (timed
+    on the library stream), then an untimed chunk-by-chunk comparison of the
+    two outputs for the max absolute error per class."""
+    import torch
+    import quartets
+    import simint
+    import workloads
+    shells = workloads.config2(natom=natom, seed=0)
+    t0 = time.perf_counter()
+    ss = quartets.ScreenedSet(shells, tol=1e-12)
+    plan64 = ss.launches
+    ss.mixed = threshold
+    ss._plan()
+    planm = ss.launches
+    setup = time.perf_counter() - t0
+    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    ref = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
+
+    def timed(plan):
+        ss.launches = plan
+        ss.run_pass(out.data_ptr(), chunk=0)      # first use of the kernels
+        simint.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t = time.perf_counter()
+        e0.record(stream)
+        ss.run_pass(out.data_ptr())
+        e1.record(stream)
+        simint.synchronize()
+        return time.perf_counter() - t, e0.elapsed_time(e1) / 1000.0
+    w64, d64 = timed(plan64)
+    wm, dm = timed(planm)
+    nmixed = sum(ln.nq for ln in planm if ln.precision == 1)
+    # error of the mixed pass against the FP64 pass, chunk by chunk (a
+    # chunk holds one class; FP64 launches of both plans are identical)
+    errs = {}
+    for c in range(ss.nchunk):
+        if not any(ln.chunk == c and ln.precision == 1 for ln in planm):
+            continue
+        n = max(int(ln.off[-1] + (ln.nket[-1] - (ln.lo[-1] if ln.lo is not None else 0))
+                    * simint.ncart(ln.cls[0]) * simint.ncart(ln.cls[1]) * simint.ncart(ln.cls[2])
+                    * simint.ncart(ln.cls[3])) for ln in planm if ln.chunk == c)
+        ss.launches = plan64
+        ss.run_pass(ref.data_ptr(), chunk=c)
+        ss.launches = planm
+        ss.run_pass(out.data_ptr(), chunk=c)
+        simint.synchronize()
+        e = float((out[:n] - ref[:n]).abs().max())
+        cls = next(ln.cls for ln in planm if ln.chunk == c)
+        name = "_".join("spdfghik"[x] for x in cls)
+        errs[name] = max(errs.get(name, 0.0), e)
+    ss.launches = planm
+    ss.free()
+    return {"workload": "%d-atom grid of the 64-atom recipe (%d shells), Schwarz 1e-12; quartets with "
+                        "sqrt((ab|ab)(cd|cd)) < %g run the FP32 VRR/HRR" % (natom, len(shells), threshold),
+            "quartets": ss.total_quartets, "fp32_quartets": nmixed,
+            "mixed_value": ss.total_quartets / wm, "fp64_value": ss.total_quartets / w64,
+            "speedup": w64 / wm, "mixed_device_s": dm, "fp64_device_s": d64, "unit": "shell quartets/s",
+            "max_abs_err_vs_fp64": max(errs.values()) if errs else 0.0,
+            "max_abs_err_per_class": errs, "bound": 1e-10, "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +214,8 @@ def main():
     ap.add_argument("--natom", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched all-classes stream")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 256-atom mixed-precision leg")
+    ap.add_argument("--mixed-natom", type=int, default=256)
     ap.add_argument("--only-class", default=None,
                     help="profiling aid: run only this class's launches (e.g. d_p_d_p)")
     ap.add_argument("--force-family", choices=["lpq", "wpq"], default=None,
@@ -283,8 +351,22 @@ def main():
     if not args.no_batched and not args.only_class:
         ss.free()
         line["batched"] = batched_leg(args.steps, args.warmup)
+    if not args.no_mixed and not args.only_class:
+        line["mixed"] = mixed_leg(args.mixed_natom)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ss)
+    # device time by (class, primitive quartets per quartet)
+    perk = {}
+    for k in range(0, len(events), 2):
+        ln, _, e0 = events[k]
+        _, _, e1 = events[k + 1]
+        key = (ln.cls, int(ln.bra.nprim * ln.ket.nprim))
+        c = perk.setdefault(key, [0.0, 0])
+        c[0] += e0.elapsed_time(e1)
+        c[1] += ln.nq
+    line["top_class_K"] = [{"class": quartets.simint_class_name(k[0]), "K": k[1], "ms_per_step": v[0] / args.steps,
+                            "quartets_per_step": v[1] // args.steps}
+                           for k, v in sorted(perk.items(), key=lambda kv: -kv[1][0])[:12]]
     top = sorted(per.items(), key=lambda kv: -kv[1]["ms"])[:8]
     line["top_classes"] = [{"class": quartets.simint_class_name(k), "ms_per_step": v["ms"] / args.steps,
                             "tflops": v["flops"] / (v["ms"] / 1000.0) / 1e12,

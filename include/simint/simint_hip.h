@@ -63,6 +63,17 @@ int  simint_hip_capture_end(simint_hip_graph ** out);
 int  simint_hip_graph_launch(simint_hip_graph * g);
 void simint_hip_graph_free(simint_hip_graph * g);
 
+/* As _prefix with a ket range per bra: kets k = ket_lo[i] .. ket_hi[i]-1
+ * (ket_lo may be NULL: from 0), quartet (b, k) written at
+ * d_out + out_off[i] + (k - ket_lo[i]) * ncart(abcd).  precision 0: FP64;
+ * 1: mixed -- the VRR and HRR in FP32 with the Boys function, the
+ * prefactors and the primitive contraction in FP64 (classes whose kernel is
+ * the wave-per-quartet family, or above d, run FP64 regardless). */
+long long simint_hip_compute_eri_range(simint_hip_pairs const * P, simint_hip_pairs const * Q,
+                                       int b0, int nbra, int const * ket_lo, int const * ket_hi,
+                                       long long const * out_off, double screen_tol, int precision,
+                                       double * d_out);
+
 /* (bb|bb) for every pair b of P, at d_out + b * ncart(abab). */
 long long simint_hip_compute_eri_diag(simint_hip_pairs const * P, double * d_out);
 

@@ -13,3 +13,6 @@ simint_launch_fn simint_dispatch(int la, int lb, int lc, int ld);
 // quartet-list launcher of the class's selected family (or the other family
 // when the selected one has no list kernel); null if the class is not built
 simint_list_launch_fn simint_dispatch_list(int la, int lb, int lc, int ld);
+// FP32-VRR/HRR prefix kernel of the class (lane-per-quartet family only);
+// null where the class has none or runs the wave-per-quartet family
+simint_launch_fn simint_dispatch_mixed(int la, int lb, int lc, int ld);

@@ -671,22 +671,27 @@ int simint_hip_pairs_count(const simint_hip_pairs *p) { return p ? p->n : 0; }
 
 namespace {
 
-// items for bra b = b0 + i: its ket prefix [0, nket[i]) in tiles of 64
-__global__ void expand_prefix_items(int b0, int nbra, const int *nket, const long long *tile_start,
-                                    const long long *out_off, WorkItem *items)
+// items for bra b = b0 + i: its ket range [lo, hi) (lo = ket_lo[i] or 0,
+// hi = nket[i]) in tiles of 64; ket k is written at out_off[i] + (k - lo) * stride
+__global__ void expand_prefix_items(int b0, int nbra, const int *ket_lo, const int *nket,
+                                    const long long *tile_start, const long long *out_off, long long stride,
+                                    WorkItem *items)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nbra) return;
-    const int nk = nket[i];
+    const int lo = ket_lo ? ket_lo[i] : 0;
+    const int hi = nket[i];
     const long long s = tile_start[i];
-    const int nt = (nk + SIMINT_TILE - 1) / SIMINT_TILE;
+    const int t0 = lo / SIMINT_TILE;
+    const int nt = hi > lo ? (hi + SIMINT_TILE - 1) / SIMINT_TILE - t0 : 0;
     for (int t = 0; t < nt; t++) {
         WorkItem w;
         w.bra = b0 + i;
-        w.tile = t;
-        w.lane_lo = 0;
-        w.lane_hi = min(SIMINT_TILE, nk - t * SIMINT_TILE);
-        w.obase = out_off[i];
+        w.tile = t0 + t;
+        w.lane_lo = max(lo - (t0 + t) * SIMINT_TILE, 0);
+        w.lane_hi = min(SIMINT_TILE, hi - (t0 + t) * SIMINT_TILE);
+        w.pad = 0;
+        w.obase = out_off[i] - (long long)lo * stride;
         items[s + t] = w;
     }
 }
@@ -695,30 +700,37 @@ __global__ void expand_prefix_items(int b0, int nbra, const int *nket, const lon
 
 extern "C" {
 
-long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
-                                        const int *nket, const long long *out_off, double screen_tol,
-                                        double *d_out)
+long long simint_hip_compute_eri_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
+                                       const int *ket_lo, const int *ket_hi, const long long *out_off,
+                                       double screen_tol, int precision, double *d_out)
 {
-    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n) return SIMINT_ERROR_ARGUMENT;
+    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n || (precision != 0 && precision != 1))
+        return SIMINT_ERROR_ARGUMENT;
     if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
-    const simint_launch_fn fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
+    simint_launch_fn fn = nullptr;
+    if (precision == 1) fn = simint_dispatch_mixed(P->am1, P->am2, Q->am1, Q->am2);
+    if (!fn) fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
     if (!fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
     if (nbra == 0) return 0;
     std::vector<long long> tstart(nbra + 1, 0);
     long long nq = 0;
     for (int i = 0; i < nbra; i++) {
-        if (nket[i] < 0 || nket[i] > Q->n) return SIMINT_ERROR_ARGUMENT;
-        tstart[i + 1] = tstart[i] + (nket[i] + SIMINT_TILE - 1) / SIMINT_TILE;
-        nq += nket[i];
+        const int lo = ket_lo ? ket_lo[i] : 0, hi = ket_hi[i];
+        if (lo < 0 || hi > Q->n) return SIMINT_ERROR_ARGUMENT;
+        const long long nt = hi > lo ? (hi + SIMINT_TILE - 1) / SIMINT_TILE - lo / SIMINT_TILE : 0;
+        tstart[i + 1] = tstart[i] + nt;
+        nq += hi > lo ? hi - lo : 0;
     }
     const long long nitems = tstart[nbra];
     if (nitems == 0) return 0;
     ThreadCtx &c = ctx();
     Image im;
+    const size_t o_lo = im.add(sizeof(int) * nbra);
     const size_t o_nk = im.add(sizeof(int) * nbra);
     const size_t o_ts = im.add(sizeof(long long) * nbra);
     const size_t o_oo = im.add(sizeof(long long) * nbra);
-    memcpy(im.at<int>(o_nk), nket, sizeof(int) * nbra);
+    if (ket_lo) memcpy(im.at<int>(o_lo), ket_lo, sizeof(int) * nbra);
+    memcpy(im.at<int>(o_nk), ket_hi, sizeof(int) * nbra);
     memcpy(im.at<long long>(o_ts), tstart.data(), sizeof(long long) * nbra);
     memcpy(im.at<long long>(o_oo), out_off, sizeof(long long) * nbra);
     hipError_t e;
@@ -734,9 +746,11 @@ long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_
     }
     char *base = static_cast<char *>(c.in.p);
     WorkItem *items = static_cast<WorkItem *>(c.items.p);
+    const long long stride = (long long)ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
     hipLaunchKernelGGL(expand_prefix_items, dim3((nbra + 255) / 256), dim3(256), 0, c.s, b0, nbra,
+                       ket_lo ? reinterpret_cast<const int *>(base + o_lo) : (const int *)nullptr,
                        reinterpret_cast<const int *>(base + o_nk), reinterpret_cast<const long long *>(base + o_ts),
-                       reinterpret_cast<const long long *>(base + o_oo), items);
+                       reinterpret_cast<const long long *>(base + o_oo), stride, items);
     if ((e = hipGetLastError()) != hipSuccess) { set_err("expand launch", e); return SIMINT_ERROR_HIP; }
     EriArgs a;
     a.bra = P->lin;
@@ -747,9 +761,19 @@ long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_
     a.boys_grid = g_dev.boys;
     a.tol2 = (screen_tol > 0.0 && P->has_screen && Q->has_screen) ? screen_tol * screen_tol : 0.0;
     a.ket_screen_max = Q->screen_max;
-    a.stride = ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
+    a.stride = stride;
     if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
     return nq;
+}
+
+long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
+                                        const int *nket, const long long *out_off, double screen_tol,
+                                        double *d_out)
+{
+    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n) return SIMINT_ERROR_ARGUMENT;
+    for (int i = 0; i < nbra; i++)
+        if (nket[i] < 0 || nket[i] > Q->n) return SIMINT_ERROR_ARGUMENT;
+    return simint_hip_compute_eri_range(P, Q, b0, nbra, nullptr, nket, out_off, screen_tol, 0, d_out);
 }
 
 long long simint_hip_compute_eri_diag(const simint_hip_pairs *P, double *d_out)

@@ -39,11 +39,16 @@ class _Writer:
         return "\n".join(self.lines) + "\n"
 
 
-def emit_body(am, w, indent="        "):
+def emit_body(am, w, indent="        ", fp="double"):
     """Emit the per-primitive-quartet code (setup, Boys, VRR, accumulation).
 
     Assumes in scope: P_alpha, P_inv, P_prefac, P_x/y/z, P_PA or P_PB (x,y,z),
-    Q_* likewise, grid, and accumulators acc<k>."""
+    Q_* likewise, grid, and accumulators acc<k> (double).  fp = "float":
+    the Gaussian-product setup, the Boys function and the prefactor stay
+    FP64; the VRR runs in FP32 on rounded copies of its inputs, and each
+    result is accumulated into the FP64 accumulators."""
+    mixed = fp == "float"
+    fm = "fmaf" if mixed else "fma"
     g = graph(*am)
     Ltot = g.Ltot
     nodes = g.node_list
@@ -78,23 +83,35 @@ def emit_body(am, w, indent="        "):
             need["2pq"].add(n.fac[2])
         if n.src[7] is not None:
             need["2pq"].add(n.fac[3])
+    cv = (lambda e: "(float)(%s)" % e) if mixed else (lambda e: e)
     if need["2p"]:
-        w(indent + "const double one_over_2p = 0.5 * P_inv;")
+        w(indent + "const %s one_over_2p = %s;" % (fp, cv("0.5 * P_inv")))
     if need["2q"]:
-        w(indent + "const double one_over_2q = 0.5 * Q_inv;")
+        w(indent + "const %s one_over_2q = %s;" % (fp, cv("0.5 * Q_inv")))
     if need["2pq"]:
-        w(indent + "const double one_over_2pq = 0.5 * one_over_PQalpha_sum;")
+        w(indent + "const %s one_over_2pq = %s;" % (fp, cv("0.5 * one_over_PQalpha_sum")))
     for kind in ("2p", "2q", "2pq"):
         for k in sorted(need[kind]):
             if k > 1:
-                w(indent + "const double c%d_%s = %s * one_over_%s;" % (k, kind, _fmt_int(k), kind))
+                w(indent + "const %s c%d_%s = %s%s * one_over_%s;" % (fp, k, kind, _fmt_int(k), "f" if mixed else "",
+                                                                     kind))
     if has_bra:
-        w(indent + "const double a_over_p = (-alpha) * P_inv;")
-        w(indent + "const double aop_PQ[3] = {a_over_p * PQx, a_over_p * PQy, a_over_p * PQz};")
+        w(indent + "const double a_over_p_d = (-alpha) * P_inv;")
+        w(indent + "const %s a_over_p = %s;" % (fp, cv("a_over_p_d")))
+        w(indent + "const %s aop_PQ[3] = {%s, %s, %s};" % (fp, cv("a_over_p_d * PQx"), cv("a_over_p_d * PQy"),
+                                                           cv("a_over_p_d * PQz")))
     if has_ket:
         w(indent + "const double a_over_q_pos = alpha * Q_inv;")
-        w(indent + "const double aoq_PQ[3] = {a_over_q_pos * PQx, a_over_q_pos * PQy, a_over_q_pos * PQz};")
-        w(indent + "const double a_over_q = -a_over_q_pos;")
+        w(indent + "const %s aoq_PQ[3] = {%s, %s, %s};" % (fp, cv("a_over_q_pos * PQx"), cv("a_over_q_pos * PQy"),
+                                                           cv("a_over_q_pos * PQz")))
+        w(indent + "const %s a_over_q = %s;" % (fp, cv("-a_over_q_pos")))
+    if mixed:
+        # FP32 copies of the pair geometry the VRR steps use
+        for side, dirs in (("P", (I, J)), ("Q", (K, L))):
+            for arr in ("PA", "PB"):
+                if any(n.step in dirs and ((arr == "PA") == (n.step in (I, K))) for n in nodes if n.step is not None):
+                    w(indent + "const float f%s_%s[3] = {(float)%s_%s[0], (float)%s_%s[1], (float)%s_%s[2]};"
+                      % (side, arr, side, arr, side, arr, side, arr))
 
     def cname(k, kind):
         return ("one_over_%s" % kind) if k == 1 else ("c%d_%s" % (k, kind))
@@ -105,39 +122,47 @@ def emit_body(am, w, indent="        "):
         v = "v%d" % len(idx)
         idx[key] = v
         if n.step is None:
-            w(indent + "const double %s = F[%d] * prefac;" % (v, n.m))
+            w(indent + "const %s %s = %s;" % (fp, v, cv("F[%d] * prefac" % n.m)))
             continue
         d = n.dir
         bra = n.step in (I, J)
+        pfx = "f" if mixed else ""
         if n.step == I:
-            pa = "P_PA[%d]" % d
+            pa = "%sP_PA[%d]" % (pfx, d)
         elif n.step == J:
-            pa = "P_PB[%d]" % d
+            pa = "%sP_PB[%d]" % (pfx, d)
         elif n.step == K:
-            pa = "Q_PA[%d]" % d
+            pa = "%sQ_PA[%d]" % (pfx, d)
         else:
-            pa = "Q_PB[%d]" % d
+            pa = "%sQ_PB[%d]" % (pfx, d)
         aop = ("aop_PQ[%d]" if bra else "aoq_PQ[%d]") % d
         aover = "a_over_p" if bra else "a_over_q"
         k2 = "2p" if bra else "2q"
         s = [idx[x] if x is not None else None for x in n.src]
         expr = "%s * %s" % (pa, s[0])
-        expr = "fma(%s, %s, %s)" % (aop, s[1], expr)
+        expr = "%s(%s, %s, %s)" % (fm, aop, s[1], expr)
         if s[2] is not None:
-            expr = "fma(%s, fma(%s, %s, %s), %s)" % (cname(n.fac[0], k2), aover, s[3], s[2], expr)
+            expr = "%s(%s, %s(%s, %s, %s), %s)" % (fm, cname(n.fac[0], k2), fm, aover, s[3], s[2], expr)
         if s[4] is not None:
-            expr = "fma(%s, fma(%s, %s, %s), %s)" % (cname(n.fac[1], k2), aover, s[5], s[4], expr)
+            expr = "%s(%s, %s(%s, %s, %s), %s)" % (fm, cname(n.fac[1], k2), fm, aover, s[5], s[4], expr)
         if s[6] is not None:
-            expr = "fma(%s, %s, %s)" % (cname(n.fac[2], "2pq"), s[6], expr)
+            expr = "%s(%s, %s, %s)" % (fm, cname(n.fac[2], "2pq"), s[6], expr)
         if s[7] is not None:
-            expr = "fma(%s, %s, %s)" % (cname(n.fac[3], "2pq"), s[7], expr)
-        w(indent + "const double %s = %s;" % (v, expr))
+            expr = "%s(%s, %s, %s)" % (fm, cname(n.fac[3], "2pq"), s[7], expr)
+        w(indent + "const %s %s = %s;" % (fp, v, expr))
     for t, key in enumerate(g.top_keys):
-        w(indent + "acc%d += %s;" % (t, idx[key]))
+        if mixed:
+            w(indent + "acc%d += (double)%s;" % (t, idx[key]))
+        else:
+            w(indent + "acc%d += %s;" % (t, idx[key]))
 
 
-def emit_hrr(am, w, indent="    ", out="o", acc="acc"):
-    """Emit the bra then ket HRR on acc<k>, storing into out[k] (Psi4 order)."""
+def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double"):
+    """Emit the bra then ket HRR on acc<k>, storing into out[k] (Psi4 order).
+    fp = "float": the HRR runs in FP32 on rounded copies of the FP64
+    contracted values and AB/CD; results are widened to FP64 on store."""
+    mixed = fp == "float"
+    fm = "fmaf" if mixed else "fma"
     g = graph(*am)
     la, lb, lc, ld = am
     bra_ops, bra_map, ket_ops, ket_map = g.hrr_program()
@@ -147,8 +172,17 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc"):
     etop = {e: i for i, e in enumerate(bra_tops)}
     ftop = {f: i for i, f in enumerate(ket_tops)}
 
+    if mixed:
+        for t in range(len(g.tops)):
+            w(indent + "const float f%s%d = (float)%s%d;" % (acc, t, acc, t))
+        if g.bra_hrr_needed():
+            w(indent + "const float fhAB[3] = {(float)hAB[0], (float)hAB[1], (float)hAB[2]};")
+        if g.ket_hrr_needed():
+            w(indent + "const float fhCD[3] = {(float)hCD[0], (float)hCD[1], (float)hCD[2]};")
+    pre = "f" if mixed else ""
+
     def topvar(e, f):
-        return "%s%d" % (acc, etop[e] * nf + ftop[f])
+        return "%s%s%d" % (pre, acc, etop[e] * nf + ftop[f])
 
     # bra HRR for every top ket component f -> names hb_<f>_<a>_<b>
     names = {}
@@ -167,8 +201,8 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc"):
         for (dst, s0, s1, sign, d) in bra_ops:
             nm = newname("hb")
             names[("b", f, dst[1], dst[2])] = nm
-            ab = ("hAB[%d]" % d) if sign > 0 else ("(-hAB[%d])" % d)
-            w(indent + "const double %s = fma(%s, %s, %s);" % (nm, ab, bra_val(s1, f), bra_val(s0, f)))
+            ab = ("%shAB[%d]" % (pre, d)) if sign > 0 else ("(-%shAB[%d])" % (pre, d))
+            w(indent + "const %s %s = %s(%s, %s, %s);" % (fp, nm, fm, ab, bra_val(s1, f), bra_val(s0, f)))
 
     # ket HRR for every final bra component
     nb, nc, nd = ncart(lb), ncart(lc), ncart(ld)
@@ -187,25 +221,39 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc"):
             for (dst, s0, s1, sign, d) in ket_ops:
                 nm = newname("hk")
                 names[("k", a, b, dst[1], dst[2])] = nm
-                cd = ("hCD[%d]" % d) if sign > 0 else ("(-hCD[%d])" % d)
-                w(indent + "const double %s = fma(%s, %s, %s);" % (nm, cd, ket_val(s1), ket_val(s0)))
+                cd = ("%shCD[%d]" % (pre, d)) if sign > 0 else ("(-%shCD[%d])" % (pre, d))
+                w(indent + "const %s %s = %s(%s, %s, %s);" % (fp, nm, fm, cd, ket_val(s1), ket_val(s0)))
             for ic, c in enumerate(carts(lc)):
                 for idd, dd in enumerate(carts(ld)):
                     k = ((ia * nb + ib) * nc + ic) * nd + idd
-                    w(indent + "%s[%d] = %s;" % (out, k, ket_val(ket_map[(c, dd)])))
+                    if mixed:
+                        w(indent + "%s[%d] = (double)%s;" % (out, k, ket_val(ket_map[(c, dd)])))
+                    else:
+                        w(indent + "%s[%d] = %s;" % (out, k, ket_val(ket_map[(c, dd)])))
 
 
-def emit_lpq(am):
-    g = graph(*am)
-    la, lb, lc, ld = am
-    name = class_name(am)
-    ntop = len(g.top_keys)
+def emit_lpq(am, mixed=False):
+    """Source of one class: the prefix kernel, the quartet-list kernel and,
+    with mixed=True, the FP32-VRR/HRR prefix kernel (eri_lpqm_<class>)."""
     w = _Writer()
-    w("// Generated by simint-generator_amd/emitter/hip_lpq.py -- class (%s)" % name.replace("_", ""))
+    w("// Generated by simint-generator_amd/emitter/hip_lpq.py -- class (%s)" % class_name(am).replace("_", ""))
     w('#include "../device_types.h"')
     w('#include "../boys_device.h"')
     w("")
-    w("__global__ void __launch_bounds__(64) eri_lpq_%s(const EriArgs A)" % name)
+    _emit_lpq_prefix(am, w, "double")
+    emit_lpq_list(am, w)
+    if mixed:
+        w("")
+        _emit_lpq_prefix(am, w, "float")
+    return w.text()
+
+
+def _emit_lpq_prefix(am, w, fp):
+    g = graph(*am)
+    name = class_name(am)
+    ntop = len(g.top_keys)
+    kname = ("eri_lpqm_%s" if fp == "float" else "eri_lpq_%s") % name
+    w("__global__ void __launch_bounds__(64) %s(const EriArgs A)" % kname)
     w("{")
     w("    const WorkItem wi = A.items[blockIdx.x];")
     w("    const int lane = threadIdx.x;")
@@ -238,7 +286,7 @@ def emit_lpq(am):
     w("            const double Q_x = A.ket.x[kj], Q_y = A.ket.y[kj], Q_z = A.ket.z[kj];")
     w("            const double Q_%s[3] = {A.ket.%sx[kj], A.ket.%sy[kj], A.ket.%sz[kj]};"
       % ("PA" if g.ket_l else "PB", ket_pa, ket_pa, ket_pa))
-    emit_body(am, w, indent="            ")
+    emit_body(am, w, indent="            ", fp=fp)
     w("        }")
     w("    }")
     w("    if (lane < wi.lane_lo || lane >= wi.lane_hi) return;")
@@ -248,18 +296,16 @@ def emit_lpq(am):
         w("    const double hAB[3] = {A.bra.abx[wi.bra], A.bra.aby[wi.bra], A.bra.abz[wi.bra]};")
     if g.ket_hrr_needed():
         w("    const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
-    emit_hrr(am, w)
+    emit_hrr(am, w, fp=fp)
     w("}")
     w("")
-    w('extern "C" hipError_t simint_launch_lpq_%s(const EriArgs *a, hipStream_t s)' % name)
+    w('extern "C" hipError_t simint_launch_%s(const EriArgs *a, hipStream_t s)' % kname[4:])
     w("{")
     w("    if (a->nitems <= 0) return hipSuccess;")
-    w("    hipLaunchKernelGGL(eri_lpq_%s, dim3((unsigned)a->nitems), dim3(64), 0, s, *a);" % name)
+    w("    hipLaunchKernelGGL(%s, dim3((unsigned)a->nitems), dim3(64), 0, s, *a);" % kname)
     w("    return hipGetLastError();")
     w("}")
     w("")
-    emit_lpq_list(am, w)
-    return w.text()
 
 
 def emit_lpq_list(am, w):

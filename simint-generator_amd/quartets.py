@@ -24,11 +24,16 @@ import simint
 
 
 class Launch:
-    __slots__ = ("bra", "ket", "b0", "nket", "off", "nq", "flops", "nout", "cls")
+    """One class launch: bras b0.. of group `bra`, kets [lo[i], nket[i]) of
+    group `ket` (lo None: from 0), quartet (b, k) at off[i] + (k - lo[i]) * nc,
+    in FP64 (precision 0) or with the FP32 VRR/HRR (1).  `chunk` numbers the
+    output-buffer chunk the launch writes (launches of one chunk share it)."""
+    __slots__ = ("bra", "ket", "b0", "nket", "off", "nq", "flops", "nout", "cls", "lo", "precision", "chunk")
 
-    def __init__(self, bra, ket, b0, nket, off, nq, flops, nout, cls):
+    def __init__(self, bra, ket, b0, nket, off, nq, flops, nout, cls, lo=None, precision=0, chunk=0):
         self.bra, self.ket, self.b0, self.nket, self.off = bra, ket, b0, nket, off
         self.nq, self.flops, self.nout, self.cls = nq, flops, nout, cls
+        self.lo, self.precision, self.chunk = lo, precision, chunk
 
 
 class Group:
@@ -54,9 +59,14 @@ def simint_class_name(am):
 
 
 class ScreenedSet:
-    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29):
+    """All unique quartets of `shells` with sqrt((ab|ab)(cd|cd)) >= tol.
+    mixed: threshold below which a quartet's bound sends it to the FP32
+    VRR/HRR kernels (None: every quartet FP64)."""
+
+    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None):
         import torch
         self.tol = tol
+        self.mixed = mixed
         self.basis = simint.Basis(shells)
         self.shells = shells
         self.out_doubles = out_doubles
@@ -97,6 +107,7 @@ class ScreenedSet:
         so that one launch's output fits the output buffer."""
         tol2 = self.tol * self.tol
         self.launches = []
+        self.nchunk = 0
         self.total_quartets = 0
         self.total_flops = 0.0
         self.total_bytes = 0.0
@@ -127,38 +138,65 @@ class ScreenedSet:
                     if nq > 0:
                         off = np.zeros(len(seg), dtype=np.int64)
                         off[1:] = np.cumsum(seg[:-1] * nc)
-                        fl = float(nq) * (K * pp + fh)
-                        self.launches.append(Launch(gb, gk, b, seg.astype(np.int32), off, nq, fl, nq * nc,
-                                                    (la, lb, lc, ld)))
+                        cls = (la, lb, lc, ld)
+                        if self.mixed is None:
+                            parts = [(None, seg, off, 0)]
+                        else:
+                            # kets are bound-sorted: FP64 for the leading
+                            # kets with bound_b * bound_k >= mixed^2
+                            with np.errstate(divide="ignore"):
+                                need64 = np.where(gb.bounds[b:e] > 0, self.mixed ** 2 / gb.bounds[b:e], np.inf)
+                            n64 = np.minimum(np.searchsorted(-kb, -need64, side="right"), seg)
+                            parts = [(None, n64, off, 0), (n64, seg, off + n64 * nc, 1)]
+                        for lo, hi, oo, prec in parts:
+                            cnt = hi - (lo if lo is not None else 0)
+                            nqp = int(cnt.sum())
+                            if nqp == 0:
+                                continue
+                            fl = float(nqp) * (K * pp + fh)
+                            self.launches.append(Launch(gb, gk, b, hi.astype(np.int32), oo, nqp, fl, nqp * nc, cls,
+                                                        None if lo is None else lo.astype(np.int32), prec,
+                                                        self.nchunk))
+                            self.total_flops += fl
+                        self.nchunk += 1
                         self.total_quartets += nq
-                        self.total_flops += fl
                         self.total_bytes += 8.0 * nq * nc
                     b = e
 
-    def run_pass(self, out_ptr, events=None):
-        """Queue every launch of one pass on the library stream.  If events is
-        a callable, it is called before and after each launch with the launch
-        (to record timing events on the library stream)."""
+    def run_pass(self, out_ptr, events=None, chunk=None):
+        """Queue every launch of one pass (or of one output chunk) on the
+        library stream.  If events is a callable, it is called before and
+        after each launch with the launch (to record timing events on the
+        library stream)."""
         for ln in self.launches:
+            if chunk is not None and ln.chunk != chunk:
+                continue
             if events:
                 events(ln, True)
-            simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out_ptr, 0.0)
+            if ln.lo is None and ln.precision == 0:
+                simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out_ptr, 0.0)
+            else:
+                simint.compute_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, 0.0,
+                                     ln.precision)
             if events:
                 events(ln, False)
 
-    def sample(self, n, seed=0):
-        """n random quartets of the screened set, as shell-index quadruples
-        with their class, weighted uniformly over quartets."""
+    def sample(self, n, seed=0, precision=None):
+        """n random quartets of the screened set (of the launches of one
+        precision if given), as shell-index quadruples, weighted uniformly
+        over quartets."""
         rng = np.random.default_rng(seed)
-        w = np.array([ln.nq for ln in self.launches], dtype=np.float64)
+        w = np.array([ln.nq if precision is None or ln.precision == precision else 0 for ln in self.launches],
+                     dtype=np.float64)
         picks = rng.choice(len(self.launches), size=n, p=w / w.sum())
         out = []
         for li in picks:
             ln = self.launches[li]
-            cum = np.cumsum(ln.nket)
+            lo = ln.lo if ln.lo is not None else np.zeros_like(ln.nket)
+            cum = np.cumsum(ln.nket - lo)
             r = int(rng.integers(0, cum[-1]))
             bi = int(np.searchsorted(cum, r, side="right"))
-            k = r - (cum[bi - 1] if bi > 0 else 0)
+            k = int(lo[bi]) + r - (cum[bi - 1] if bi > 0 else 0)
             bp = ln.bra.pairs[ln.b0 + bi]
             kp = ln.ket.pairs[k]
             out.append((bp[0], bp[1], kp[0], kp[1]))

@@ -56,7 +56,7 @@ EXPORTS = [
     "simint_hip_pairs_create", "simint_hip_pairs_free", "simint_hip_pairs_count",
     "simint_hip_compute_eri_prefix", "simint_hip_compute_eri_diag", "simint_hip_synchronize",
     "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
-    "simint_hip_compute_eri_list", "simint_hip_deriv1_worksize", "simint_hip_compute_eri_deriv1_list",
+    "simint_hip_compute_eri_list", "simint_hip_compute_eri_range", "simint_hip_deriv1_worksize", "simint_hip_compute_eri_deriv1_list",
     "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
 ]
 
@@ -100,6 +100,9 @@ def lib():
         L.simint_hip_compute_eri_list.argtypes = [c_void_p, c_void_p, ll, c_void_p, c_void_p, c_void_p, c_double,
                                                   c_void_p]
         L.simint_hip_compute_eri_list.restype = ll
+        L.simint_hip_compute_eri_range.argtypes = [c_void_p, c_void_p, c_int, c_int, iptr, iptr, llptr, c_double,
+                                                   c_int, c_void_p]
+        L.simint_hip_compute_eri_range.restype = ll
         L.simint_hip_deriv1_worksize.argtypes = [c_void_p, c_void_p, ll]
         L.simint_hip_deriv1_worksize.restype = ll
         L.simint_hip_compute_eri_deriv1_list.argtypes = [c_void_p, c_void_p, ll, c_void_p, c_void_p, c_void_p,
@@ -299,6 +302,21 @@ class _Capture:
             raise RuntimeError("simint_hip_capture_end failed (%d): %s" % (r, last_error()))
         self.g.h = h
         return False
+
+
+def compute_range(P, Q, b0, ket_lo, ket_hi, out_off, d_out_ptr, screen_tol=0.0, precision=0):
+    """simint_hip_compute_eri_range: kets [ket_lo[i], ket_hi[i]) of bra b0+i
+    (ket_lo None: from 0); precision 1 = FP32 VRR/HRR where available."""
+    ket_hi = np.ascontiguousarray(ket_hi, dtype=np.int32)
+    lo = None if ket_lo is None else np.ascontiguousarray(ket_lo, dtype=np.int32)
+    out_off = np.ascontiguousarray(out_off, dtype=np.int64)
+    r = lib().simint_hip_compute_eri_range(P.h, Q.h, int(b0), len(ket_hi),
+                                           None if lo is None else lo.ctypes.data_as(iptr),
+                                           ket_hi.ctypes.data_as(iptr), out_off.ctypes.data_as(llptr),
+                                           float(screen_tol), int(precision), c_void_p(d_out_ptr))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_eri_range failed (%d): %s" % (r, last_error()))
+    return r
 
 
 def compute_prefix(P, Q, b0, nket, out_off, d_out_ptr, screen_tol=0.0):
