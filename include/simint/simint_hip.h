@@ -54,6 +54,14 @@ long long simint_hip_compute_eri_deriv1_list(simint_hip_pairs const * P, simint_
                                              long long const * d_qout, double screen_tol, double * d_work,
                                              double * d_out);
 
+/* First derivatives over ket ranges (as _range, FP64): quartet (b0+i, k),
+ * k in [ket_lo[i], ket_hi[i]), written densely in (bra, ket) order -- quartet
+ * number sum_{j<i}(ket_hi[j]-ket_lo[j]) + (k - ket_lo[i]) at d_out + number *
+ * 12 ncart(abcd).  d_work: simint_hip_deriv1_worksize(P, Q, nquartet) doubles. */
+long long simint_hip_compute_eri_deriv1_range(simint_hip_pairs const * P, simint_hip_pairs const * Q,
+                                              int b0, int nbra, int const * ket_lo, int const * ket_hi,
+                                              double screen_tol, double * d_work, double * d_out);
+
 /* Graph capture of the calling thread's library stream: every list call
  * between begin and end is recorded (not run); simint_hip_graph_launch
  * replays the whole batch as one graph launch. */
@@ -82,9 +90,11 @@ void *simint_hip_stream(void);                /* hipStream_t of the calling thre
 const char *simint_hip_last_error(void);
 
 /* Kernel family used for a class: 1 = lane-per-quartet (registers),
- * 2 = wave-per-quartet (LDS-staged), 0 = class not built.  The setter lets a
- * caller (the autotuner, tests) pick a family; it returns the previous one or
- * a negative value if that family is not built for the class. */
+ * 2 = wave-per-quartet (LDS-staged), 3 / 4 = split lane-per-quartet (the
+ * contracted HRR inputs processed in groups of about 64 / 128, through a
+ * device staging area), 0 = class not built.  The setter lets a caller (the
+ * autotuner, tests) pick a family; it returns the previous one or a
+ * negative value if that family is not built for the class. */
 int simint_hip_class_kernel(int la, int lb, int lc, int ld);
 int simint_hip_set_class_kernel(int la, int lb, int lc, int ld, int family);
 

@@ -35,10 +35,17 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I" + INCLUDE, "-I" + CSRC, "-Wall"]
 sys.path.insert(0, PKG)
 
 
-def _header_hash():
+def _header_hash(kernels_only=False):
+    """Hash of the headers a source depends on: the generated class kernels
+    include only the device layout, the Boys code and its tables; the
+    runtime includes everything."""
     h = hashlib.sha1()
-    files = sorted(glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(GEN, "*.inc"))
-                   + glob.glob(os.path.join(INCLUDE, "simint", "**", "*.h*"), recursive=True))
+    if kernels_only:
+        files = sorted([os.path.join(CSRC, "device_types.h"), os.path.join(CSRC, "boys_device.h")]
+                       + glob.glob(os.path.join(GEN, "*.inc")))
+    else:
+        files = sorted(glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(GEN, "*.inc"))
+                       + glob.glob(os.path.join(INCLUDE, "simint", "**", "*.h*"), recursive=True))
     for f in files:
         h.update(f.encode())
         with open(f, "rb") as fh:
@@ -103,19 +110,21 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
         d = json.load(open(okf))
         if d.get("maxam") == maxam:
             lpq_ok = set(tuple(x) for x in d["ok"])
-    lpq, wpq, sel = gen_mod.generate(PKG, INCLUDE, maxam, lpq_ok)
+    lpq, wpq, sel, lpqs = gen_mod.generate(PKG, INCLUDE, maxam, lpq_ok)
     hdr = _header_hash()
     srcs = sorted(glob.glob(os.path.join(GEN, "eri_*.hip")))
     # drop stale generated kernels of classes outside this build
     from emitter.hip_lpq import class_name
     want = set("eri_lpq_%s.hip" % class_name(a) for a in lpq) | set("eri_wpq_%s.hip" % class_name(a) for a in wpq)
+    want |= set("eri_lpqs%d_%s.hip" % (G, class_name(a)) for a, Gs in lpqs.items() for G in Gs)
     srcs = [s for s in srcs if os.path.basename(s) in want]
     rt = [os.path.join(CSRC, "runtime.hip"), os.path.join(CSRC, "shell.cpp"), os.path.join(CSRC, "shell_screen.cpp")]
     reports = {}
     objs = {}
     nbuilt = 0
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = {ex.submit(_compile, s, hdr, "eri_lpq_" in s): s for s in srcs + rt}
+        khdr = _header_hash(kernels_only=True)
+        futs = {ex.submit(_compile, s, khdr if s in srcs else hdr, "eri_lpq_" in s): s for s in srcs + rt}
         for f in cf.as_completed(futs):
             s = futs[f]
             obj, rep, built = f.result()
@@ -130,7 +139,7 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
             ok.add(am)
     with open(okf, "w") as f:
         json.dump({"maxam": maxam, "ok": sorted(list(a) for a in ok)}, f)
-    lpq, wpq, sel = gen_mod.generate(PKG, INCLUDE, maxam, ok)
+    lpq, wpq, sel, lpqs = gen_mod.generate(PKG, INCLUDE, maxam, ok)
     hdr = _header_hash()
     dobj, _, built = _compile_dispatch(hdr)
     objs["dispatch"] = dobj

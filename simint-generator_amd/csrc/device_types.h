@@ -84,4 +84,5 @@ struct EriArgs {
     double tol2;                   // primitive screening threshold (squared); <= 0 disables
     double ket_screen_max;         // simint_multi_shellpair.screen_max of the ket set
     long long stride;              // doubles per quartet in the output
+    double *scratch;               // per-work-item staging (split kernels only)
 };

@@ -10,6 +10,9 @@ typedef hipError_t (*simint_launch_fn)(const EriArgs *, hipStream_t);
 typedef hipError_t (*simint_list_launch_fn)(const EriListArgs *, hipStream_t);
 
 simint_launch_fn simint_dispatch(int la, int lb, int lc, int ld);
+// staging doubles per lane the selected kernel needs in EriArgs::scratch
+// (split lane-per-quartet family), 0 for the others
+int simint_dispatch_staging(int la, int lb, int lc, int ld);
 // quartet-list launcher of the class's selected family (or the other family
 // when the selected one has no list kernel); null if the class is not built
 simint_list_launch_fn simint_dispatch_list(int la, int lb, int lc, int ld);

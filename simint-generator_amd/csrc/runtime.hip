@@ -120,6 +120,7 @@ struct PinBuf {
 struct ThreadCtx {
     hipStream_t s = nullptr;
     DevBuf in, out, items;
+    DevBuf split;      // staging of the split lane-per-quartet kernels
     PinBuf stage;
     ThreadCtx() { hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
 };
@@ -552,11 +553,18 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
     e = hipMemcpyAsync(c.in.p, c.stage.p, im.buf.size(), hipMemcpyHostToDevice, c.s);
     if (e != hipSuccess) { set_err("hipMemcpyAsync(H2D)", e); return SIMINT_ERROR_HIP; }
 
+    size_t nst = 0;
+    for (auto &t : terms) nst = std::max(nst, (size_t)simint_dispatch_staging(t.am[0], t.am[1], t.am[2], t.am[3]));
+    if (nst && (e = c.split.reserve(sizeof(double) * nst * SIMINT_TILE * (size_t)nit)) != hipSuccess) {
+        set_err("hipMalloc", e);
+        return SIMINT_ERROR_HIP;
+    }
     char *base = static_cast<char *>(c.in.p);
     double *dout = static_cast<double *>(c.out.p);
     for (size_t k = 0; k < terms.size(); k++) {
         const Term &t = terms[k];
         EriArgs a;
+        a.scratch = static_cast<double *>(c.split.p);
         a.bra = linear_ptrs(base, lo);
         a.ket = tiled_ptrs(base, to);
         if (t.weight && t.centre == 0) a.bra.prefac = reinterpret_cast<const double *>(base + lo.prefac_a2);
@@ -623,6 +631,7 @@ struct simint_hip_pairs {
     PairsTiled til;
     int ntile;
     const double *wa = nullptr, *wb = nullptr;   // linear prefac * alpha2, prefac * beta2
+    const double *twa = nullptr, *twb = nullptr; // the same in the tiled view
 };
 
 int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **out)
@@ -636,7 +645,7 @@ int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **
     std::vector<int> order(n);
     std::iota(order.begin(), order.end(), 0);
     std::vector<long long> colmap(order.begin(), order.end());
-    const TiledOffs to = pack_tiled(im, P, order, colmap);
+    const TiledOffs to = pack_tiled(im, P, order, colmap, true);
     void *d = nullptr;
     hipError_t e = hipMalloc(&d, im.buf.size());
     if (e != hipSuccess) { set_err("hipMalloc(pairs)", e); return SIMINT_ERROR_HIP; }
@@ -654,6 +663,8 @@ int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **
     p->ntile = to.ntile;
     p->wa = reinterpret_cast<const double *>(static_cast<char *>(d) + lo.prefac_a2);
     p->wb = reinterpret_cast<const double *>(static_cast<char *>(d) + lo.prefac_b2);
+    p->twa = reinterpret_cast<const double *>(static_cast<char *>(d) + to.prefac_a2);
+    p->twb = reinterpret_cast<const double *>(static_cast<char *>(d) + to.prefac_b2);
     *out = p;
     return 0;
 }
@@ -698,19 +709,21 @@ __global__ void expand_prefix_items(int b0, int nbra, const int *ket_lo, const i
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-long long simint_hip_compute_eri_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
-                                       const int *ket_lo, const int *ket_hi, const long long *out_off,
-                                       double screen_tol, int precision, double *d_out)
+// One range launch of class `am` over the pairs' primitive data (which do not
+// depend on the AM the class kernel is compiled for); with `weight` 0..3 the
+// 2a-weighted prefactors of that centre replace the plain ones.
+struct RangeTerm {
+    int am[4];
+    simint_launch_fn fn;
+    int weight;   // -1: none
+};
+
+long long range_launch(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra, const int *ket_lo,
+                       const int *ket_hi, const long long *const *out_offs, const RangeTerm *terms, int nterm,
+                       double screen_tol, double *const *d_outs)
 {
-    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n || (precision != 0 && precision != 1))
-        return SIMINT_ERROR_ARGUMENT;
-    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
-    simint_launch_fn fn = nullptr;
-    if (precision == 1) fn = simint_dispatch_mixed(P->am1, P->am2, Q->am1, Q->am2);
-    if (!fn) fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
-    if (!fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
     if (nbra == 0) return 0;
     std::vector<long long> tstart(nbra + 1, 0);
     long long nq = 0;
@@ -728,42 +741,81 @@ long long simint_hip_compute_eri_range(const simint_hip_pairs *P, const simint_h
     const size_t o_lo = im.add(sizeof(int) * nbra);
     const size_t o_nk = im.add(sizeof(int) * nbra);
     const size_t o_ts = im.add(sizeof(long long) * nbra);
-    const size_t o_oo = im.add(sizeof(long long) * nbra);
+    std::vector<size_t> o_oo(nterm);
+    for (int t = 0; t < nterm; t++) o_oo[t] = im.add(sizeof(long long) * nbra);
     if (ket_lo) memcpy(im.at<int>(o_lo), ket_lo, sizeof(int) * nbra);
     memcpy(im.at<int>(o_nk), ket_hi, sizeof(int) * nbra);
     memcpy(im.at<long long>(o_ts), tstart.data(), sizeof(long long) * nbra);
-    memcpy(im.at<long long>(o_oo), out_off, sizeof(long long) * nbra);
+    for (int t = 0; t < nterm; t++) memcpy(im.at<long long>(o_oo[t]), out_offs[t], sizeof(long long) * nbra);
     hipError_t e;
     // the previous call's kernels may still read c.in / c.items: wait for them
     if ((e = hipStreamSynchronize(c.s)) != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
     if ((e = c.stage.reserve(im.buf.size())) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
     if ((e = c.in.reserve(im.buf.size())) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
-    if ((e = c.items.reserve(sizeof(WorkItem) * (size_t)nitems)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    if ((e = c.items.reserve(sizeof(WorkItem) * (size_t)nitems * nterm)) != hipSuccess) {
+        set_err("hipMalloc", e);
+        return SIMINT_ERROR_HIP;
+    }
+    size_t nst = 0;
+    for (int t = 0; t < nterm; t++)
+        nst = std::max(nst, (size_t)simint_dispatch_staging(terms[t].am[0], terms[t].am[1], terms[t].am[2],
+                                                            terms[t].am[3]));
+    if (nst && (e = c.split.reserve(sizeof(double) * nst * SIMINT_TILE * (size_t)nitems)) != hipSuccess) {
+        set_err("hipMalloc", e);
+        return SIMINT_ERROR_HIP;
+    }
     memcpy(c.stage.p, im.buf.data(), im.buf.size());
     if ((e = hipMemcpyAsync(c.in.p, c.stage.p, im.buf.size(), hipMemcpyHostToDevice, c.s)) != hipSuccess) {
         set_err("hipMemcpyAsync", e);
         return SIMINT_ERROR_HIP;
     }
     char *base = static_cast<char *>(c.in.p);
-    WorkItem *items = static_cast<WorkItem *>(c.items.p);
-    const long long stride = (long long)ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
-    hipLaunchKernelGGL(expand_prefix_items, dim3((nbra + 255) / 256), dim3(256), 0, c.s, b0, nbra,
-                       ket_lo ? reinterpret_cast<const int *>(base + o_lo) : (const int *)nullptr,
-                       reinterpret_cast<const int *>(base + o_nk), reinterpret_cast<const long long *>(base + o_ts),
-                       reinterpret_cast<const long long *>(base + o_oo), stride, items);
-    if ((e = hipGetLastError()) != hipSuccess) { set_err("expand launch", e); return SIMINT_ERROR_HIP; }
-    EriArgs a;
-    a.bra = P->lin;
-    a.ket = Q->til;
-    a.items = items;
-    a.nitems = nitems;
-    a.out = d_out;
-    a.boys_grid = g_dev.boys;
-    a.tol2 = (screen_tol > 0.0 && P->has_screen && Q->has_screen) ? screen_tol * screen_tol : 0.0;
-    a.ket_screen_max = Q->screen_max;
-    a.stride = stride;
-    if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    for (int t = 0; t < nterm; t++) {
+        const RangeTerm &T = terms[t];
+        WorkItem *items = static_cast<WorkItem *>(c.items.p) + (size_t)nitems * t;
+        const long long stride = (long long)ncart(T.am[0]) * ncart(T.am[1]) * ncart(T.am[2]) * ncart(T.am[3]);
+        hipLaunchKernelGGL(expand_prefix_items, dim3((nbra + 255) / 256), dim3(256), 0, c.s, b0, nbra,
+                           ket_lo ? reinterpret_cast<const int *>(base + o_lo) : (const int *)nullptr,
+                           reinterpret_cast<const int *>(base + o_nk),
+                           reinterpret_cast<const long long *>(base + o_ts),
+                           reinterpret_cast<const long long *>(base + o_oo[t]), stride, items);
+        if ((e = hipGetLastError()) != hipSuccess) { set_err("expand launch", e); return SIMINT_ERROR_HIP; }
+        EriArgs a;
+        a.scratch = static_cast<double *>(c.split.p);
+        a.bra = P->lin;
+        a.ket = Q->til;
+        if (T.weight == 0) a.bra.prefac = P->wa;
+        if (T.weight == 1) a.bra.prefac = P->wb;
+        if (T.weight == 2) a.ket.prefac = Q->twa;
+        if (T.weight == 3) a.ket.prefac = Q->twb;
+        a.items = items;
+        a.nitems = nitems;
+        a.out = d_outs[t];
+        a.boys_grid = g_dev.boys;
+        a.tol2 = (screen_tol > 0.0 && P->has_screen && Q->has_screen) ? screen_tol * screen_tol : 0.0;
+        a.ket_screen_max = Q->screen_max;
+        a.stride = stride;
+        if ((e = T.fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+    }
     return nq;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long simint_hip_compute_eri_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
+                                       const int *ket_lo, const int *ket_hi, const long long *out_off,
+                                       double screen_tol, int precision, double *d_out)
+{
+    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n || (precision != 0 && precision != 1))
+        return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    RangeTerm t{{P->am1, P->am2, Q->am1, Q->am2}, nullptr, -1};
+    if (precision == 1) t.fn = simint_dispatch_mixed(P->am1, P->am2, Q->am1, Q->am2);
+    if (!t.fn) t.fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
+    if (!t.fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    return range_launch(P, Q, b0, nbra, ket_lo, ket_hi, &out_off, &t, 1, screen_tol, &d_out);
 }
 
 long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
@@ -797,12 +849,18 @@ long long simint_hip_compute_eri_diag(const simint_hip_pairs *P, double *d_out)
     const size_t bytes = sizeof(WorkItem) * items.size();
     if ((e = c.stage.reserve(bytes)) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
     if ((e = c.items.reserve(bytes)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
+    const size_t nst = (size_t)simint_dispatch_staging(P->am1, P->am2, P->am1, P->am2);
+    if (nst && (e = c.split.reserve(sizeof(double) * nst * SIMINT_TILE * items.size())) != hipSuccess) {
+        set_err("hipMalloc", e);
+        return SIMINT_ERROR_HIP;
+    }
     memcpy(c.stage.p, items.data(), bytes);
     if ((e = hipMemcpyAsync(c.items.p, c.stage.p, bytes, hipMemcpyHostToDevice, c.s)) != hipSuccess) {
         set_err("hipMemcpyAsync", e);
         return SIMINT_ERROR_HIP;
     }
     EriArgs a;
+    a.scratch = static_cast<double *>(c.split.p);
     a.bra = P->lin;
     a.ket = P->til;
     a.items = static_cast<const WorkItem *>(c.items.p);
@@ -815,6 +873,8 @@ long long simint_hip_compute_eri_diag(const simint_hip_pairs *P, double *d_out)
     if ((e = fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
     return P->n;
 }
+
+}  // extern "C"
 
 // ------------------------------------------------------------ quartet lists
 namespace {
@@ -873,6 +933,8 @@ EriListArgs list_args(const simint_hip_pairs *P, const simint_hip_pairs *Q, long
 }
 
 }  // namespace
+
+extern "C" {
 
 long long simint_hip_compute_eri_list(const simint_hip_pairs *P, const simint_hip_pairs *Q, long long nquartet,
                                       const int *d_qbra, const int *d_qket, const long long *d_qout,
@@ -948,6 +1010,61 @@ long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const si
     d.qout = d_qout;
     if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
     return nquartet;
+}
+
+long long simint_hip_compute_eri_deriv1_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0,
+                                              int nbra, const int *ket_lo, const int *ket_hi, double screen_tol,
+                                              double *d_work, double *d_out)
+{
+    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n || !d_work || !d_out) return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
+    const DerivPlan dp = deriv1_plan(am);
+    RangeTerm terms[8];
+    for (int k = 0; k < dp.nt; k++) {
+        const DerivPlan::T &t = dp.t[k];
+        terms[k] = RangeTerm{{t.am[0], t.am[1], t.am[2], t.am[3]}, nullptr, t.weight ? t.centre : -1};
+        if (t.am[0] <= SIMINT_OSTEI_MAXAM && t.am[1] <= SIMINT_OSTEI_MAXAM && t.am[2] <= SIMINT_OSTEI_MAXAM &&
+            t.am[3] <= SIMINT_OSTEI_MAXAM)
+            terms[k].fn = simint_dispatch(t.am[0], t.am[1], t.am[2], t.am[3]);
+        if (!terms[k].fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    }
+    // dense quartet numbering in (bra, ket) order
+    std::vector<long long> qpre(nbra + 1, 0);
+    for (int i = 0; i < nbra; i++) {
+        const int lo = ket_lo ? ket_lo[i] : 0, hi = ket_hi[i];
+        if (lo < 0 || hi > Q->n) return SIMINT_ERROR_ARGUMENT;
+        qpre[i + 1] = qpre[i] + (hi > lo ? hi - lo : 0);
+    }
+    const long long nq = qpre[nbra];
+    if (nq == 0) return 0;
+    std::vector<std::vector<long long>> offs(dp.nt, std::vector<long long>(nbra));
+    const long long *offp[8];
+    double *outs[8];
+    for (int k = 0; k < dp.nt; k++) {
+        for (int i = 0; i < nbra; i++) offs[k][i] = qpre[i] * dp.t[k].nc;
+        offp[k] = offs[k].data();
+        outs[k] = d_work + (long long)dp.t[k].off * nq;
+    }
+    long long r = range_launch(P, Q, b0, nbra, ket_lo, ket_hi, offp, terms, dp.nt, screen_tol, outs);
+    if (r < 0) return r;
+    Deriv1Args d;
+    for (int q = 0; q < 4; q++) {
+        d.am[q] = am[q];
+        d.plus[q] = nullptr;
+        d.minus[q] = nullptr;
+    }
+    for (int k = 0; k < dp.nt; k++) {
+        if (dp.t[k].sign > 0) d.plus[dp.t[k].centre] = outs[k];
+        else d.minus[dp.t[k].centre] = outs[k];
+    }
+    d.skip = dp.skip;
+    d.nquartet = nq;
+    d.out = d_out;
+    d.qout = nullptr;
+    hipError_t e = launch_deriv1_assemble(d, ctx().s);
+    if (e != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
+    return nq;
 }
 
 // ------------------------------------------------------------ graph capture

@@ -39,7 +39,7 @@ class _Writer:
         return "\n".join(self.lines) + "\n"
 
 
-def emit_body(am, w, indent="        ", fp="double"):
+def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     """Emit the per-primitive-quartet code (setup, Boys, VRR, accumulation).
 
     Assumes in scope: P_alpha, P_inv, P_prefac, P_x/y/z, P_PA or P_PB (x,y,z),
@@ -51,7 +51,13 @@ def emit_body(am, w, indent="        ", fp="double"):
     fm = "fmaf" if mixed else "fma"
     g = graph(*am)
     Ltot = g.Ltot
-    nodes = g.node_list
+    # keep: the VRR nodes to emit (a closure of acc_tops); acc_tops: list of
+    # (accumulator name, top key) -- by default every node and every top
+    nodes = g.node_list if keep is None else [n for n in g.node_list if (n.q, n.m) in keep]
+    if acc_tops is None:
+        acc_tops = [("acc%d" % t, key) for t, key in enumerate(g.top_keys)]
+    else:
+        Ltot = max(n.m for n in nodes if n.step is None)
     used_steps = set(n.step for n in nodes if n.step is not None)
     has_bra = bool(used_steps & {I, J})
     has_ket = bool(used_steps & {K, L})
@@ -150,11 +156,11 @@ def emit_body(am, w, indent="        ", fp="double"):
         if s[7] is not None:
             expr = "%s(%s, %s, %s)" % (fm, cname(n.fac[3], "2pq"), s[7], expr)
         w(indent + "const %s %s = %s;" % (fp, v, expr))
-    for t, key in enumerate(g.top_keys):
+    for accname, key in acc_tops:
         if mixed:
-            w(indent + "acc%d += (double)%s;" % (t, idx[key]))
+            w(indent + "%s += (double)%s;" % (accname, idx[key]))
         else:
-            w(indent + "acc%d += %s;" % (t, idx[key]))
+            w(indent + "%s += %s;" % (accname, idx[key]))
 
 
 def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double"):
@@ -234,7 +240,7 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double"):
 
 def emit_lpq(am, mixed=False):
     """Source of one class: the prefix kernel, the quartet-list kernel and,
-    with mixed=True, the FP32-VRR/HRR prefix kernel (eri_lpqm_<class>)."""
+    with mixed=True, the FP32-VRR prefix kernel (eri_lpqm_<class>)."""
     w = _Writer()
     w("// Generated by simint-generator_amd/emitter/hip_lpq.py -- class (%s)" % class_name(am).replace("_", ""))
     w('#include "../device_types.h"')
@@ -248,7 +254,7 @@ def emit_lpq(am, mixed=False):
     return w.text()
 
 
-def _emit_lpq_prefix(am, w, fp):
+def _emit_lpq_prefix(am, w, fp, hrr_fp="double"):
     g = graph(*am)
     name = class_name(am)
     ntop = len(g.top_keys)
@@ -296,7 +302,10 @@ def _emit_lpq_prefix(am, w, fp):
         w("    const double hAB[3] = {A.bra.abx[wi.bra], A.bra.aby[wi.bra], A.bra.abz[wi.bra]};")
     if g.ket_hrr_needed():
         w("    const double hCD[3] = {A.ket.abx[tl], A.ket.aby[tl], A.ket.abz[tl]};")
-    emit_hrr(am, w, fp=fp)
+    # the HRR stays FP64: its AB / CD transfers cancel strongly for the
+    # far-apart pairs that have small Schwarz bounds, and FP32 there loses
+    # far more than the ERI error budget
+    emit_hrr(am, w, fp=hrr_fp)
     w("}")
     w("")
     w('extern "C" hipError_t simint_launch_%s(const EriArgs *a, hipStream_t s)' % kname[4:])

@@ -63,10 +63,11 @@ class ScreenedSet:
     mixed: threshold below which a quartet's bound sends it to the FP32
     VRR/HRR kernels (None: every quartet FP64)."""
 
-    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None):
+    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None, deriv=0):
         import torch
         self.tol = tol
         self.mixed = mixed
+        self.deriv = deriv
         self.basis = simint.Basis(shells)
         self.shells = shells
         self.out_doubles = out_doubles
@@ -117,6 +118,8 @@ class ScreenedSet:
                 la, lb = gb.cls
                 lc, ld = gk.cls
                 pp, fh, nc = _class_flops(self.info, la, lb, lc, ld)
+                if self.deriv:
+                    nc *= 12
                 # kets surviving the screen: bound_b * bound_k >= tol^2
                 kb = gk.bounds
                 with np.errstate(divide="ignore"):
@@ -163,17 +166,26 @@ class ScreenedSet:
                         self.total_bytes += 8.0 * nq * nc
                     b = e
 
-    def run_pass(self, out_ptr, events=None, chunk=None):
+    def deriv_worksize(self):
+        """Device scratch (doubles) the largest derivative launch needs."""
+        return max([simint.deriv1_worksize(ln.bra.dev, ln.ket.dev, ln.nq) for ln in self.launches] + [1])
+
+    def run_pass(self, out_ptr, events=None, chunk=None, work_ptr=None):
         """Queue every launch of one pass (or of one output chunk) on the
         library stream.  If events is a callable, it is called before and
         after each launch with the launch (to record timing events on the
         library stream)."""
+        self.work_ptr = work_ptr
+        if self.deriv and not work_ptr:
+            raise ValueError("a derivative pass needs a work buffer (deriv_worksize doubles)")
         for ln in self.launches:
             if chunk is not None and ln.chunk != chunk:
                 continue
             if events:
                 events(ln, True)
-            if ln.lo is None and ln.precision == 0:
+            if self.deriv:
+                simint.compute_deriv1_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, self.work_ptr, out_ptr)
+            elif ln.lo is None and ln.precision == 0:
                 simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out_ptr, 0.0)
             else:
                 simint.compute_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, 0.0,

@@ -56,7 +56,8 @@ EXPORTS = [
     "simint_hip_pairs_create", "simint_hip_pairs_free", "simint_hip_pairs_count",
     "simint_hip_compute_eri_prefix", "simint_hip_compute_eri_diag", "simint_hip_synchronize",
     "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
-    "simint_hip_compute_eri_list", "simint_hip_compute_eri_range", "simint_hip_deriv1_worksize", "simint_hip_compute_eri_deriv1_list",
+    "simint_hip_compute_eri_list", "simint_hip_compute_eri_range", "simint_hip_deriv1_worksize",
+    "simint_hip_compute_eri_deriv1_range", "simint_hip_compute_eri_deriv1_list",
     "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
 ]
 
@@ -103,6 +104,9 @@ def lib():
         L.simint_hip_compute_eri_range.argtypes = [c_void_p, c_void_p, c_int, c_int, iptr, iptr, llptr, c_double,
                                                    c_int, c_void_p]
         L.simint_hip_compute_eri_range.restype = ll
+        L.simint_hip_compute_eri_deriv1_range.argtypes = [c_void_p, c_void_p, c_int, c_int, iptr, iptr, c_double,
+                                                          c_void_p, c_void_p]
+        L.simint_hip_compute_eri_deriv1_range.restype = ll
         L.simint_hip_deriv1_worksize.argtypes = [c_void_p, c_void_p, ll]
         L.simint_hip_deriv1_worksize.restype = ll
         L.simint_hip_compute_eri_deriv1_list.argtypes = [c_void_p, c_void_p, ll, c_void_p, c_void_p, c_void_p,
@@ -316,6 +320,19 @@ def compute_range(P, Q, b0, ket_lo, ket_hi, out_off, d_out_ptr, screen_tol=0.0, 
                                            float(screen_tol), int(precision), c_void_p(d_out_ptr))
     if r < 0:
         raise RuntimeError("simint_hip_compute_eri_range failed (%d): %s" % (r, last_error()))
+    return r
+
+
+def compute_deriv1_range(P, Q, b0, ket_lo, ket_hi, d_work, d_out_ptr, screen_tol=0.0):
+    """simint_hip_compute_eri_deriv1_range: output dense in (bra, ket) order."""
+    ket_hi = np.ascontiguousarray(ket_hi, dtype=np.int32)
+    lo = None if ket_lo is None else np.ascontiguousarray(ket_lo, dtype=np.int32)
+    r = lib().simint_hip_compute_eri_deriv1_range(P.h, Q.h, int(b0), len(ket_hi),
+                                                  None if lo is None else lo.ctypes.data_as(iptr),
+                                                  ket_hi.ctypes.data_as(iptr), float(screen_tol),
+                                                  c_void_p(d_work), c_void_p(d_out_ptr))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_eri_deriv1_range failed (%d): %s" % (r, last_error()))
     return r
 
 

@@ -194,8 +194,9 @@ def test_deriv1_screened_and_limits(gpu, screen):
 
 
 def test_lane_and_wave_kernels_agree(gpu):
-    """Both kernel families (lane-per-quartet in registers, wave-per-quartet
-    in LDS) against the oracle for every class that has both."""
+    """Every kernel family (lane-per-quartet in registers, wave-per-quartet
+    in LDS, the split lane-per-quartet kernels) against the oracle for every
+    class up to d that has it."""
     shells = workloads.config2(natom=2)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
@@ -206,7 +207,7 @@ def test_lane_and_wave_kernels_agree(gpu):
     checked = 0
     for am in itertools.product(range(3), repeat=4):
         prev = L.simint_hip_class_kernel(*am)
-        for fam in (1, 2):
+        for fam in (1, 2, 3, 4):
             if L.simint_hip_set_class_kernel(*am, fam) < 0:
                 continue
             _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]])
