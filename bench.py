@@ -51,6 +51,28 @@ def measured_traffic(kernel):
     return None
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print("[bench %.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
+
+
+T_START = time.perf_counter()
+BATCH_STREAMS = 8    # side streams the batched stream's classes are spread over
+
+
+def kernel_symbol(info, cls):
+    """Name of the kernel the class runs now (the family selected at run
+    time; the split kernels carry their group count)."""
+    import simint
+    name = "_".join("spdfghik"[x] for x in cls)
+    fam = simint.lib().simint_hip_class_kernel(*cls)
+    if fam == 1:
+        return "eri_lpq_%s" % name
+    if fam in (3, 4):
+        return "eri_lpqs%d_%s" % (info[name]["split_groups"][fam - 3], name)
+    return "eri_wpq_%s" % name
+
+
 def cpu_baseline(ss, seconds=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -88,6 +110,7 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
     basis = simint.Basis(shells)
     b = batch.QuartetBatch(basis, q)
     setup = time.perf_counter() - t0
+    progress("batched stream set up (%.1f s)" % setup)
     flops = 0.0
     for g in b.groups:
         c = info["_".join("spdfghik"[x] for x in g.cls)]
@@ -103,12 +126,13 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
         ev.append((g, e))
     b.run(out.data_ptr(), events=rec)
     simint.synchronize()
+    progress("batched: eager pass with per-class events done")
     per = []
     for k in range(0, len(ev), 2):
         g, e0 = ev[k]
         _, e1 = ev[k + 1]
         per.append((e0.elapsed_time(e1), g))
-    gr = b.capture(out.data_ptr())
+    gr = b.capture(out.data_ptr(), streams=BATCH_STREAMS)
     for _ in range(warmup):
         gr.launch()
     simint.synchronize()
@@ -131,17 +155,18 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
            "ms_per_pass": 1000.0 * wall / steps, "device_ms_per_pass": 1000.0 * dev_s / steps,
            "tflops": flops * steps / dev_s / 1e12, "frac_fp64_peak": flops * steps / dev_s / PEAK_FP64,
            "flops_per_pass": flops, "output_bytes": 8.0 * b.out_doubles,
-           "launch": "one hipGraph per pass holding %d quartet-list class launches" % len(b.groups),
+           "launch": "one hipGraph per pass holding %d quartet-list class launches on %d forked streams"
+                     % (len(b.groups), BATCH_STREAMS),
            "setup_s": setup,
            "top_classes": [{"class": "_".join("spdfghik"[x] for x in g.cls), "ms": ms, "quartets": g.n,
-                            "kernel": ("lpq" if simint.lib().simint_hip_class_kernel(*g.cls) == 1 else "wpq")}
+                            "kernel": kernel_symbol(info, g.cls)}
                            for ms, g in per[:6]]}
     b.free()
     return res
 
 
-def mixed_leg(natom=256, threshold=1e-4):
-    """The 256-atom system of the 64-atom recipe: every screened quartet
+def mixed_leg(natom=64, threshold=1e-4):
+    """A system of the 64-atom recipe (64 atoms by default, 256 on request): every screened quartet
     (Schwarz 1e-12) in FP64, then again with quartets whose bound is below
     `threshold` run through the FP32-VRR/HRR kernels; one pass each (timed
     on the library stream), then an untimed chunk-by-chunk comparison of the
@@ -158,6 +183,7 @@ def mixed_leg(natom=256, threshold=1e-4):
     ss._plan()
     planm = ss.launches
     setup = time.perf_counter() - t0
+    progress("%d-atom set: %d quartets (setup %.1f s)" % (natom, ss.total_quartets, setup))
     out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
     ref = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
     stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
@@ -175,24 +201,29 @@ def mixed_leg(natom=256, threshold=1e-4):
         simint.synchronize()
         return time.perf_counter() - t, e0.elapsed_time(e1) / 1000.0
     w64, d64 = timed(plan64)
+    progress("FP64 pass %.1f s" % w64)
     wm, dm = timed(planm)
+    progress("mixed pass %.1f s" % wm)
     nmixed = sum(ln.nq for ln in planm if ln.precision == 1)
     # error of the mixed pass against the FP64 pass, chunk by chunk (a
     # chunk holds one class; FP64 launches of both plans are identical)
     errs = {}
+    t_last = time.perf_counter()
+    ss.launches = planm
     for c in range(ss.nchunk):
-        if not any(ln.chunk == c and ln.precision == 1 for ln in planm):
+        if time.perf_counter() - t_last > 20.0:
+            progress("error check: chunk %d of %d" % (c, ss.nchunk))
+            t_last = time.perf_counter()
+        if not any(ln.precision == 1 for ln in ss.chunk_launches(c)):
             continue
-        n = max(int(ln.off[-1] + (ln.nket[-1] - (ln.lo[-1] if ln.lo is not None else 0))
-                    * simint.ncart(ln.cls[0]) * simint.ncart(ln.cls[1]) * simint.ncart(ln.cls[2])
-                    * simint.ncart(ln.cls[3])) for ln in planm if ln.chunk == c)
+        n = ss.chunk_doubles(c)
         ss.launches = plan64
         ss.run_pass(ref.data_ptr(), chunk=c)
         ss.launches = planm
         ss.run_pass(out.data_ptr(), chunk=c)
         simint.synchronize()
         e = float((out[:n] - ref[:n]).abs().max())
-        cls = next(ln.cls for ln in planm if ln.chunk == c)
+        cls = ss.chunk_launches(c)[0].cls
         name = "_".join("spdfghik"[x] for x in cls)
         errs[name] = max(errs.get(name, 0.0), e)
     ss.launches = planm
@@ -215,10 +246,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched all-classes stream")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 256-atom mixed-precision leg")
-    ap.add_argument("--mixed-natom", type=int, default=256)
+    ap.add_argument("--mixed-natom", type=int, default=64,
+                    help="atoms of the mixed-precision leg (the 256-atom system takes ~7 minutes)")
+    ap.add_argument("--tuning", default=None, help="apply this kernel-family tuning file before running")
     ap.add_argument("--only-class", default=None,
                     help="profiling aid: run only this class's launches (e.g. d_p_d_p)")
-    ap.add_argument("--force-family", choices=["lpq", "wpq"], default=None,
+    ap.add_argument("--force-family", choices=["lpq", "wpq", "lpqs_a", "lpqs_b"], default=None,
                     help="profiling aid: use this kernel family for every class that has it")
     ap.add_argument("--copy-gib", type=float, default=None,
                     help="profiling aid: run one device copy of this many GiB and exit "
@@ -246,8 +279,10 @@ def main():
     import workloads
 
     shells = workloads.config2(natom=args.natom, seed=0)
+    if args.tuning:
+        simint.apply_tuning(args.tuning)
     if args.force_family:
-        fam = {"lpq": 1, "wpq": 2}[args.force_family]
+        fam = simint.FAMILIES[args.force_family]
         for a in range(3):
             for b in range(3):
                 for c in range(3):
@@ -256,6 +291,8 @@ def main():
     t_setup = time.perf_counter()
     ss = quartets.ScreenedSet(shells, tol=1e-12)
     t_setup = time.perf_counter() - t_setup
+    progress("64-atom set: %d quartets in %d launches (setup %.1f s)" % (ss.total_quartets, len(ss.launches),
+                                                                         t_setup))
     if args.only_class:
         ss.launches = [ln for ln in ss.launches if quartets.simint_class_name(ln.cls) == args.only_class]
         ss.total_quartets = sum(ln.nq for ln in ss.launches)
@@ -268,6 +305,7 @@ def main():
         ss.run_pass(out.data_ptr())
     simint.synchronize()
     torch.cuda.synchronize()
+    progress("warmup done")
 
     events = []
 
@@ -285,6 +323,7 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     ms_per_step = 1000.0 * elapsed / args.steps
+    progress("timed steps done: %.1f ms per step" % ms_per_step)
     value = ss.total_quartets * args.steps / elapsed
 
     # per-class device time from the events
@@ -314,8 +353,7 @@ def main():
         achieved = bytes_per_launch / mean_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": achieved / (PEAK_HBM / 1e9), "traffic": None}
-    info = ss.info[quartets.simint_class_name(dom)]
-    kname = "eri_%s_%s" % (info["kernel"], quartets.simint_class_name(dom))
+    kname = kernel_symbol(ss.info, dom)
     roof["traffic"] = measured_traffic(kname)
     roof.update({"kernel": kname,
                  "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -351,10 +389,13 @@ def main():
     if not args.no_batched and not args.only_class:
         ss.free()
         line["batched"] = batched_leg(args.steps, args.warmup)
+        progress("batched leg done")
     if not args.no_mixed and not args.only_class:
         line["mixed"] = mixed_leg(args.mixed_natom)
+        progress("mixed leg done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ss)
+        progress("cpu baseline done")
     # device time by (class, primitive quartets per quartet)
     perk = {}
     for k in range(0, len(events), 2):
@@ -371,7 +412,7 @@ def main():
     line["top_classes"] = [{"class": quartets.simint_class_name(k), "ms_per_step": v["ms"] / args.steps,
                             "tflops": v["flops"] / (v["ms"] / 1000.0) / 1e12,
                             "quartets_per_step": v["quartets"] // args.steps,
-                            "kernel": ss.info[quartets.simint_class_name(k)]["kernel"]} for k, v in top]
+                            "kernel": kernel_symbol(ss.info, k)} for k, v in top]
     print(json.dumps(line))
     if args.no_batched or args.only_class:
         ss.free()

@@ -74,9 +74,10 @@ void simint_hip_graph_free(simint_hip_graph * g);
 /* As _prefix with a ket range per bra: kets k = ket_lo[i] .. ket_hi[i]-1
  * (ket_lo may be NULL: from 0), quartet (b, k) written at
  * d_out + out_off[i] + (k - ket_lo[i]) * ncart(abcd).  precision 0: FP64;
- * 1: mixed -- the VRR and HRR in FP32 with the Boys function, the
- * prefactors and the primitive contraction in FP64 (classes whose kernel is
- * the wave-per-quartet family, or above d, run FP64 regardless). */
+ * 1: mixed -- the VRR in FP32, with the Boys function, the prefactors, the
+ * primitive contraction and the HRR in FP64 (classes whose selected kernel is
+ * not the lane-per-quartet family, or above d, run FP64 regardless).  The
+ * caller decides which quartets may take FP32 (their Schwarz bound). */
 long long simint_hip_compute_eri_range(simint_hip_pairs const * P, simint_hip_pairs const * Q,
                                        int b0, int nbra, int const * ket_lo, int const * ket_hi,
                                        long long const * out_off, double screen_tol, int precision,
@@ -87,6 +88,14 @@ long long simint_hip_compute_eri_diag(simint_hip_pairs const * P, double * d_out
 
 int   simint_hip_synchronize(void);
 void *simint_hip_stream(void);                /* hipStream_t of the calling thread */
+/* Queue the calling thread's later calls on `stream` (a hipStream_t; NULL:
+ * back to the thread's own stream), e.g. to spread the classes of one batch
+ * over several streams that a graph capture joins.  The calls that stage
+ * host data through the library's per-thread buffers (_prefix, _range,
+ * _diag, the drop-in calls) synchronise only the current stream before
+ * reusing those buffers: synchronise the previous stream yourself before
+ * switching between such calls; list calls take device arrays and need no care. */
+int   simint_hip_set_stream(void * stream);
 const char *simint_hip_last_error(void);
 
 /* Kernel family used for a class: 1 = lane-per-quartet (registers),

@@ -89,9 +89,15 @@ class QuartetBatch:
             self.groups.append(g)
         self.work = torch.empty(max(work, 1), dtype=torch.float64, device="cuda") if deriv else None
 
-    def run(self, d_out, events=None):
+    def run(self, d_out, events=None, streams=None):
         """Queue every class on the library stream (asynchronous).  events(g,
-        start) is called before and after each class launch if given."""
+        start) is called before and after each class launch if given.
+        streams: spread the classes round-robin over this many side streams,
+        forked from and joined back into the library stream (classes with
+        few quartets each fill only part of the GPU; side by side they fill
+        it).  Inside a capture the fork and join become graph edges."""
+        if streams:
+            return self._run_streams(d_out, streams)
         for g in self.groups:
             if events:
                 events(g, True)
@@ -104,10 +110,48 @@ class QuartetBatch:
             if events:
                 events(g, False)
 
-    def capture(self, d_out):
+    def _prepare(self, n):
+        """Side streams and per-stream derivative scratch, allocated before
+        any capture (no allocation may happen while one records)."""
+        import torch
+        if getattr(self, "_side", None) is None or len(self._side) != n:
+            self._side = [torch.cuda.Stream() for _ in range(n)]
+            self._works = [self.work] + [torch.empty_like(self.work) for _ in range(n - 1)] if self.deriv else None
+
+    def _run_streams(self, d_out, n):
+        import torch
+        L = simint.lib()
+        self._prepare(n)
+        main = torch.cuda.ExternalStream(L.simint_hip_stream())
+        fork = torch.cuda.Event()
+        fork.record(main)
+        for s in self._side:
+            s.wait_event(fork)
+        # largest classes first, so the long launches start early
+        order = sorted(self.groups, key=lambda g: -g.prim_quartets * (1 + sum(g.cls)) ** 3)
+        try:
+            for k, g in enumerate(order):
+                L.simint_hip_set_stream(ctypes.c_void_p(self._side[k % n].cuda_stream))
+                if self.deriv:
+                    simint.compute_deriv1_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(),
+                                               g.qout.data_ptr(), self._works[k % n].data_ptr(), d_out,
+                                               self.screen_tol)
+                else:
+                    simint.compute_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(),
+                                        d_out, self.screen_tol)
+        finally:
+            L.simint_hip_set_stream(None)
+        for s in self._side:
+            e = torch.cuda.Event()
+            e.record(s)
+            main.wait_event(e)
+
+    def capture(self, d_out, streams=None):
         """The whole batch recorded into one graph (replay with .launch())."""
+        if streams:
+            self._prepare(streams)
         with simint.Graph.capture() as gr:
-            self.run(d_out)
+            self.run(d_out, streams=streams)
         return gr
 
     def free(self):

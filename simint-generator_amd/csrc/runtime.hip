@@ -118,11 +118,16 @@ struct PinBuf {
 // resources from thread-exit or static destructors can race the runtime's
 // own teardown, and the process exit returns the memory anyway.
 struct ThreadCtx {
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr;       // the stream calls of this thread queue on
+    hipStream_t own = nullptr;     // this thread's own stream (s unless redirected)
     DevBuf in, out, items;
     DevBuf split;      // staging of the split lane-per-quartet kernels
     PinBuf stage;
-    ThreadCtx() { hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+    ThreadCtx()
+    {
+        hipStreamCreateWithFlags(&own, hipStreamNonBlocking);
+        s = own;
+    }
 };
 
 ThreadCtx &ctx()
@@ -1127,6 +1132,14 @@ void *simint_hip_stream(void)
 {
     if (!ensure_device()) return nullptr;
     return (void *)ctx().s;
+}
+
+int simint_hip_set_stream(void *stream)
+{
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    ThreadCtx &c = ctx();
+    c.s = stream ? static_cast<hipStream_t>(stream) : c.own;
+    return 0;
 }
 
 }  // extern "C"

@@ -163,10 +163,28 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
             w(indent + "%s += %s;" % (accname, idx[key]))
 
 
-def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double"):
+STAGE = 16   # output components per LDS slice (coalesced stores)
+
+
+def _emit_flush(w, indent, k0, sl):
+    """Write components [k0, k0 + sl) of the wave's quartets from the LDS
+    slice: consecutive threads store consecutive doubles of one quartet's
+    block, so every store instruction writes sl-double runs instead of 64
+    scattered doubles."""
+    w(indent + "__syncthreads();")
+    w(indent + "for (int e = lane; e < %d; e += 64) {" % (64 * sl))
+    w(indent + "    const int q = e / %d, j = e - q * %d;" % (sl, sl))
+    w(indent + "    if (q >= wi.lane_lo && q < wi.lane_hi) A.out[ob[q] + %d + j] = xb[q * %d + j];" % (k0, STAGE + 1))
+    w(indent + "}")
+    w(indent + "__syncthreads();")
+
+
+def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double", staged=False):
     """Emit the bra then ket HRR on acc<k>, storing into out[k] (Psi4 order).
     fp = "float": the HRR runs in FP32 on rounded copies of the FP64
-    contracted values and AB/CD; results are widened to FP64 on store."""
+    contracted values and AB/CD; results are widened to FP64 on store.
+    staged: results go through the LDS slices xb / ob (see _emit_flush)
+    instead of straight to out[k]; every lane of the wave must run it."""
     mixed = fp == "float"
     fm = "fmaf" if mixed else "fma"
     g = graph(*am)
@@ -212,6 +230,7 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double"):
 
     # ket HRR for every final bra component
     nb, nc, nd = ncart(lb), ncart(lc), ncart(ld)
+    ntot = ncart(la) * nb * nc * nd
     for ia, a in enumerate(carts(la)):
         for ib, b in enumerate(carts(lb)):
             bm = bra_map[(a, b)]
@@ -232,10 +251,15 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double"):
             for ic, c in enumerate(carts(lc)):
                 for idd, dd in enumerate(carts(ld)):
                     k = ((ia * nb + ib) * nc + ic) * nd + idd
+                    val = ket_val(ket_map[(c, dd)])
                     if mixed:
-                        w(indent + "%s[%d] = (double)%s;" % (out, k, ket_val(ket_map[(c, dd)])))
+                        val = "(double)" + val
+                    if staged:
+                        w(indent + "xb[lane * %d + %d] = %s;" % (STAGE + 1, k % STAGE, val))
+                        if (k + 1) % STAGE == 0 or k + 1 == ntot:
+                            _emit_flush(w, indent, k - k % STAGE, k % STAGE + 1)
                     else:
-                        w(indent + "%s[%d] = %s;" % (out, k, ket_val(ket_map[(c, dd)])))
+                        w(indent + "%s[%d] = %s;" % (out, k, val))
 
 
 def emit_lpq(am, mixed=False):
@@ -295,6 +319,8 @@ def _emit_lpq_prefix(am, w, fp, hrr_fp="double"):
     emit_body(am, w, indent="            ", fp=fp)
     w("        }")
     w("    }")
+    # direct stores (through LDS slices measured 3% slower on these
+    # compute-bound kernels)
     w("    if (lane < wi.lane_lo || lane >= wi.lane_hi) return;")
     w("    const long long colv = A.ket.col[tl];")
     w("    double *__restrict__ o = A.out + wi.obase + colv * A.stride;")

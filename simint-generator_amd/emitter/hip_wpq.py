@@ -164,7 +164,8 @@ def emit_wpq(am):
     return w.text()
 
 
-QCH = 64   # quartets per workgroup in the quartet-list kernels
+# quartets per workgroup in the quartet-list kernels: one batch of B, so
+# that a class with few quartets still spreads over many workgroups
 
 
 def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
@@ -203,8 +204,8 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
         w("        const long long tl = (long long)wi.tile * SIMINT_TILE + ql;")
     else:
         w("    __shared__ int kk[B];")
-        w("    const long long qbeg = (long long)blockIdx.x * %d;" % QCH)
-        w("    const long long qend = (qbeg + %d) < A.nquartet ? (qbeg + %d) : A.nquartet;" % (QCH, QCH))
+        w("    const long long qbeg = (long long)blockIdx.x * B;")
+        w("    const long long qend = (qbeg + B) < A.nquartet ? (qbeg + B) : A.nquartet;")
         w("    for (long long q0 = qbeg; q0 < qend; q0 += B) {")
         w("        const int nb = (qend - q0) < B ? (int)(qend - q0) : B;")
         w("        const long long qi = (q0 + b) < qend ? (q0 + b) : (qend - 1);")
@@ -341,8 +342,8 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
         w('extern "C" hipError_t simint_launch_wpql_%s(const EriListArgs *a, hipStream_t s)' % name)
         w("{")
         w("    if (a->nquartet <= 0) return hipSuccess;")
-        w("    hipLaunchKernelGGL(eri_wpql_%s, dim3((unsigned)((a->nquartet + %d) / %d)), dim3(%d), 0, s, *a);"
-          % (name, QCH - 1, QCH, THREADS))
+        w("    hipLaunchKernelGGL(eri_wpql_%s, dim3((unsigned)((a->nquartet + B - 1) / B)), dim3(%d), 0, s, *a);"
+          % (name, THREADS))
         w("    return hipGetLastError();")
         w("}")
 

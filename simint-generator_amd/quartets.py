@@ -58,10 +58,30 @@ def simint_class_name(am):
     return "_".join("spdfghik"[x] for x in am)
 
 
+def mixed_threshold(cls, base=1e-4):
+    """Schwarz bound below which a quartet of class cls runs the FP32-VRR
+    kernel.  The FP32 rounding error of the VRR outputs is relative to their
+    size; the HRR transfers (a,b+1| = (a+1,b| + AB (a,b| cancel strongly for
+    the far-apart pairs that have small bounds and amplify it, by about 40x
+    with one HRR side and about 10^4 with both (measured on shells of the
+    64-atom recipe).  The base threshold applies to classes without HRR;
+    the others get it divided by 400 and 10^5, which keeps the absolute
+    error of every class at or below 1e-10 on the 256-atom system (measured:
+    1.9e-10 with divisors 100 and 2.5e4)."""
+    la, lb, lc, ld = cls
+    bra_hrr = min(la, lb) > 0
+    ket_hrr = min(lc, ld) > 0
+    if bra_hrr and ket_hrr:
+        return base / 1e5
+    if bra_hrr or ket_hrr:
+        return base / 400.0
+    return base
+
+
 class ScreenedSet:
     """All unique quartets of `shells` with sqrt((ab|ab)(cd|cd)) >= tol.
-    mixed: threshold below which a quartet's bound sends it to the FP32
-    VRR/HRR kernels (None: every quartet FP64)."""
+    mixed: base threshold (see mixed_threshold) below which a quartet's
+    bound sends it to the FP32-VRR kernels (None: every quartet FP64)."""
 
     def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None, deriv=0):
         import torch
@@ -147,8 +167,9 @@ class ScreenedSet:
                         else:
                             # kets are bound-sorted: FP64 for the leading
                             # kets with bound_b * bound_k >= mixed^2
-                            with np.errstate(divide="ignore"):
-                                need64 = np.where(gb.bounds[b:e] > 0, self.mixed ** 2 / gb.bounds[b:e], np.inf)
+                            thr = mixed_threshold(cls, self.mixed)
+                            with np.errstate(divide="ignore", over="ignore"):
+                                need64 = np.where(gb.bounds[b:e] > 0, thr ** 2 / gb.bounds[b:e], np.inf)
                             n64 = np.minimum(np.searchsorted(-kb, -need64, side="right"), seg)
                             parts = [(None, n64, off, 0), (n64, seg, off + n64 * nc, 1)]
                         for lo, hi, oo, prec in parts:
@@ -166,6 +187,25 @@ class ScreenedSet:
                         self.total_bytes += 8.0 * nq * nc
                     b = e
 
+    def chunk_launches(self, chunk):
+        """Launches writing output chunk `chunk` (index built on first use)."""
+        cache = self.__dict__.setdefault("_by_chunk", {})
+        ent = cache.get(id(self.launches))
+        if ent is None or ent[0] is not self.launches:
+            by = {}
+            for ln in self.launches:
+                by.setdefault(ln.chunk, []).append(ln)
+            ent = cache[id(self.launches)] = (self.launches, by)
+        return ent[1].get(chunk, [])
+
+    def chunk_doubles(self, chunk):
+        """Extent of output chunk `chunk` in doubles."""
+        n = 0
+        for ln in self.chunk_launches(chunk):
+            lo = 0 if ln.lo is None else int(ln.lo[-1])
+            n = max(n, int(ln.off[-1]) + (int(ln.nket[-1]) - lo) * (ln.nout // max(ln.nq, 1)))
+        return n
+
     def deriv_worksize(self):
         """Device scratch (doubles) the largest derivative launch needs."""
         return max([simint.deriv1_worksize(ln.bra.dev, ln.ket.dev, ln.nq) for ln in self.launches] + [1])
@@ -178,9 +218,7 @@ class ScreenedSet:
         self.work_ptr = work_ptr
         if self.deriv and not work_ptr:
             raise ValueError("a derivative pass needs a work buffer (deriv_worksize doubles)")
-        for ln in self.launches:
-            if chunk is not None and ln.chunk != chunk:
-                continue
+        for ln in (self.launches if chunk is None else self.chunk_launches(chunk)):
             if events:
                 events(ln, True)
             if self.deriv:

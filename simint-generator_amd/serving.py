@@ -13,8 +13,8 @@ ERIs (or first derivatives) of one synthetic molecule built with the
     thread), so their kernels overlap on the device;
   * keeps each request's shell pairs resident in HBM while it runs, streams
     its integrals chunk by chunk through one output buffer, and reduces every
-    chunk on the device into a digest (sum of values, sum of squares, max |x|
-    in FP64), the stand-in for a consumer such as a Fock build.
+    chunk on the device into a digest (sum of values and sum of squares in
+    FP64), the stand-in for a consumer such as a Fock build.
 
 Latency is arrival to completion; throughput is shell quartets (derivative
 quartets counted once) per second of wall time.
@@ -82,7 +82,8 @@ def footprint(natom, kind, chunk_doubles):
 
 
 class Server:
-    def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 25):
+    def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False):
+        self.verbose = verbose
         self.budget = float(hbm_budget)
         self.free = float(hbm_budget)
         self.slots = slots
@@ -134,6 +135,10 @@ class Server:
                 self.free += fp
                 self.running -= 1
                 self.cv.notify_all()
+            if self.verbose:
+                print("[serve %.1fs] request %d (%d atoms, %s) done: %.2f s after arrival, %d quartets%s"
+                      % (r.t_done, r.rid, r.natom, r.kind, r.t_done - r.arrival, r.quartets,
+                         "" if r.error is None else ", error " + r.error), file=sys.stderr, flush=True)
             done.put(r)
 
     def _process(self, r, out, stream):
@@ -144,16 +149,14 @@ class Server:
         work = None
         if deriv:
             work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
-        s = torch.zeros(3, dtype=torch.float64, device="cuda")
+        s = torch.zeros(2, dtype=torch.float64, device="cuda")
         with torch.cuda.stream(stream):
             for c in range(ss.nchunk):
-                n = max(int(ln.off[-1] + int(ln.nket[-1]) * (ln.nout // max(ln.nq, 1)))
-                        for ln in ss.launches if ln.chunk == c)
+                n = ss.chunk_doubles(c)
                 ss.run_pass(out.data_ptr(), chunk=c, work_ptr=None if work is None else work.data_ptr())
                 x = out[:n]
                 s[0] += x.sum()
-                s[1] += (x * x).sum()
-                s[2] = torch.maximum(s[2], x.abs().max())
+                s[1] += torch.dot(x, x)
             stream.synchronize()
         r.quartets = ss.total_quartets
         r.digest = [float(v) for v in s.cpu()]
@@ -210,7 +213,7 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=64.0)
     a = ap.parse_args()
     reqs = request_stream(a.requests, a.rate, a.seed, (a.min_atoms, a.max_atoms))
-    srv = Server(hbm_budget=a.budget_gib * (1 << 30), slots=a.slots, tol=a.tol)
+    srv = Server(hbm_budget=a.budget_gib * (1 << 30), slots=a.slots, tol=a.tol, verbose=True)
     srv.serve(reqs)
     res = summarize(reqs, srv.wall)
     res["max_concurrent"] = srv.max_concurrent

@@ -57,7 +57,7 @@ EXPORTS = [
     "simint_hip_compute_eri_prefix", "simint_hip_compute_eri_diag", "simint_hip_synchronize",
     "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
     "simint_hip_compute_eri_list", "simint_hip_compute_eri_range", "simint_hip_deriv1_worksize",
-    "simint_hip_compute_eri_deriv1_range", "simint_hip_compute_eri_deriv1_list",
+    "simint_hip_compute_eri_deriv1_range", "simint_hip_set_stream", "simint_hip_compute_eri_deriv1_list",
     "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
 ]
 
@@ -116,6 +116,7 @@ def lib():
         L.simint_hip_graph_launch.argtypes = [c_void_p]
         L.simint_hip_graph_free.argtypes = [c_void_p]
         L.simint_hip_stream.restype = c_void_p
+        L.simint_hip_set_stream.argtypes = [c_void_p]
         L.simint_hip_last_error.restype = ctypes.c_char_p
         L.simint_hip_class_kernel.argtypes = [c_int] * 4
         L.simint_hip_set_class_kernel.argtypes = [c_int] * 5
@@ -242,6 +243,21 @@ class DevicePairs:
         if self.h:
             lib().simint_hip_pairs_free(self.h)
             self.h = None
+
+
+FAMILIES = {"lpq": 1, "wpq": 2, "lpqs_a": 3, "lpqs_b": 4}
+
+
+def apply_tuning(path):
+    """Set each class's kernel family from a tuning file (as autotune.py
+    writes it) at run time; returns the number of classes changed."""
+    import json
+    n = 0
+    for name, fam in json.load(open(path)).get("family", {}).items():
+        am = ["spdfghik".index(c) for c in name.split("_")]
+        if lib().simint_hip_set_class_kernel(*am, FAMILIES[fam]) >= 0:
+            n += 1
+    return n
 
 
 def compute_list(P, Q, n, d_qbra, d_qket, d_qout, d_out, screen_tol=0.0):

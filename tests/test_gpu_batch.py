@@ -111,17 +111,25 @@ def test_list_graph_replay(gpu):
     eager = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
     b.run(eager.data_ptr())
     simint.synchronize()
-    out = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
-    g = b.capture(out.data_ptr())
-    for rep in range(3):
+    for streams in (None, 4):
+        out = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
+        g = b.capture(out.data_ptr(), streams=streams)
+        for rep in range(3):
+            out.zero_()
+            torch.cuda.synchronize()
+            g.launch()
+            simint.synchronize()
+            assert torch.equal(out, eager), (streams, rep)
+            b.run(eager.data_ptr())          # eager work between replays
+            simint.synchronize()
+        g.free()
+        # eager, spread over side streams
         out.zero_()
         torch.cuda.synchronize()
-        g.launch()
+        b.run(out.data_ptr(), streams=streams)
         simint.synchronize()
-        assert torch.equal(out, eager), rep
-        b.run(eager.data_ptr())          # eager work between replays
-        simint.synchronize()
-    g.free()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), streams
     b.free()
 
 

@@ -309,3 +309,42 @@ def test_device_prefix_and_diag(gpu):
             refd, offs, sizes = oracle.eri(sa, [p + p for p in pairsP])
             assert parity_report(outd.cpu().numpy(), refd, offs, sizes)[0]
         P.free(); Q.free(); Pp.free(); Qp.free()
+
+
+def test_split_kernels_at_scale(gpu):
+    """Thousands of work items in flight at once (the split kernels' staging
+    areas must not overlap): (bb|bb) of every (dp) and (dd) pair of a 16-atom
+    system through each family, equal to the oracle on a sample and to each
+    other everywhere."""
+    import torch
+    shells = workloads.config2(natom=16, seed=0)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    L = simint.lib()
+    rng = np.random.default_rng(5)
+    for cls in [(2, 1), (2, 2)]:
+        pairs = [(i, j) for i in range(len(shells)) for j in range(len(shells))
+                 if shells[i][0] == cls[0] and shells[j][0] == cls[1] and (cls[0] != cls[1] or i >= j)]
+        tmp = simint.Pairs.from_pairs(basis, pairs)
+        dev = simint.DevicePairs(tmp)
+        nab = simint.ncart(cls[0]) * simint.ncart(cls[1])
+        am = (cls[0], cls[1], cls[0], cls[1])
+        prev = L.simint_hip_class_kernel(*am)
+        res = {}
+        for fam in (1, 2, 3, 4):
+            if L.simint_hip_set_class_kernel(*am, fam) < 0:
+                continue
+            out = torch.full((len(pairs) * nab * nab,), float("nan"), dtype=torch.float64, device="cuda")
+            simint.compute_diag(dev, out.data_ptr())
+            simint.synchronize()
+            res[fam] = out.cpu().numpy()
+        L.simint_hip_set_class_kernel(*am, prev)
+        sel = rng.choice(len(pairs), size=24, replace=False)
+        ref, offs, sizes = oracle.eri(sa, [pairs[k] + pairs[k] for k in sel])
+        for fam, v in res.items():
+            got = np.concatenate([v[k * nab * nab:(k + 1) * nab * nab] for k in sel])
+            ok, st = parity_report(got, ref, offs, sizes)
+            assert ok, (cls, fam, st)
+            assert np.abs(v - res[2]).max() <= 1e-13 * max(1.0, np.abs(res[2]).max()), (cls, fam)
+        dev.free()
+        tmp.free()
