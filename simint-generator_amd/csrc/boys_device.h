@@ -58,7 +58,9 @@ __device__ __forceinline__ void boys_eval(const double *__restrict__ grid, doubl
     } else {
         // top order from the long-range form, then down (boys.h:49-80,
         // boys_long.h:29-35): this keeps the reference's values, whose
-        // e^-x tail enters through the recursion
+        // e^-x tail enters through the recursion.  pow, not a product of
+        // powers of x^-1/2: the latter carries ~(2L+1) ulp, and high-AM
+        // classes amplify input errors far above the reference's own.
         F[L] = longfac[L] * sqrt(pow(x, -(double)(2 * L + 1)));
         const double x2 = 2.0 * x;
         const double ex = exp(-x);
