@@ -149,43 +149,69 @@ class ScreenedSet:
                 if gk is gb:
                     nket = np.minimum(nket, np.arange(1, len(gb.bounds) + 1))
                 K = gb.nprim * gk.nprim
-                # chunk bras so output per launch <= out_doubles
+                cls = (la, lb, lc, ld)
+                cap = self.out_doubles
+                # chunks: runs of whole bras whose output fits `cap`; a bra
+                # whose own kets exceed it is cut into ket ranges
                 b = 0
                 nb = len(nket)
                 while b < nb:
+                    if nket[b] * nc > cap:
+                        step = max(1, cap // nc)
+                        for k0 in range(0, int(nket[b]), step):
+                            k1 = min(k0 + step, int(nket[b]))
+                            self._emit_chunk(gb, gk, cls, K, pp, fh, nc, kb, b, b + 1,
+                                             np.array([k0], dtype=np.int64), np.array([k1], dtype=np.int64))
+                        b += 1
+                        continue
                     cum = np.cumsum(nket[b:] * nc)
-                    e = b + max(1, int(np.searchsorted(cum, self.out_doubles, side="right")))
-                    e = min(e, nb)
-                    seg = nket[b:e]
-                    nq = int(seg.sum())
-                    if nq > 0:
-                        off = np.zeros(len(seg), dtype=np.int64)
-                        off[1:] = np.cumsum(seg[:-1] * nc)
-                        cls = (la, lb, lc, ld)
-                        if self.mixed is None:
-                            parts = [(None, seg, off, 0)]
-                        else:
-                            # kets are bound-sorted: FP64 for the leading
-                            # kets with bound_b * bound_k >= mixed^2
-                            thr = mixed_threshold(cls, self.mixed)
-                            with np.errstate(divide="ignore", over="ignore"):
-                                need64 = np.where(gb.bounds[b:e] > 0, thr ** 2 / gb.bounds[b:e], np.inf)
-                            n64 = np.minimum(np.searchsorted(-kb, -need64, side="right"), seg)
-                            parts = [(None, n64, off, 0), (n64, seg, off + n64 * nc, 1)]
-                        for lo, hi, oo, prec in parts:
-                            cnt = hi - (lo if lo is not None else 0)
-                            nqp = int(cnt.sum())
-                            if nqp == 0:
-                                continue
-                            fl = float(nqp) * (K * pp + fh)
-                            self.launches.append(Launch(gb, gk, b, hi.astype(np.int32), oo, nqp, fl, nqp * nc, cls,
-                                                        None if lo is None else lo.astype(np.int32), prec,
-                                                        self.nchunk))
-                            self.total_flops += fl
-                        self.nchunk += 1
-                        self.total_quartets += nq
-                        self.total_bytes += 8.0 * nq * nc
+                    e = min(nb, b + int(np.searchsorted(cum, cap, side="right")))
+                    self._emit_chunk(gb, gk, cls, K, pp, fh, nc, kb, b, e, None, nket[b:e])
                     b = e
+        self._check_plan()
+
+    def _emit_chunk(self, gb, gk, cls, K, pp, fh, nc, kb, b, e, lo, hi):
+        """Launches of one output chunk: bras b..e-1, kets [lo, hi) each
+        (lo None: from 0), laid out densely from the chunk start."""
+        lo0 = np.zeros_like(hi) if lo is None else lo
+        seg = hi - lo0
+        nq = int(seg.sum())
+        if nq <= 0:
+            return
+        off = np.zeros(len(seg), dtype=np.int64)
+        off[1:] = np.cumsum(seg[:-1] * nc)
+        if self.mixed is None:
+            parts = [(lo, hi, off, 0)]
+        else:
+            # kets are bound-sorted: FP64 for the leading kets with
+            # bound_b * bound_k >= threshold^2
+            thr = mixed_threshold(cls, self.mixed)
+            with np.errstate(divide="ignore", over="ignore"):
+                need64 = np.where(gb.bounds[b:e] > 0, thr ** 2 / gb.bounds[b:e], np.inf)
+            n64 = np.clip(np.searchsorted(-kb, -need64, side="right"), lo0, hi)
+            parts = [(lo, n64, off, 0), (n64, hi, off + (n64 - lo0) * nc, 1)]
+        for plo, phi, oo, prec in parts:
+            cnt = phi - (plo if plo is not None else 0)
+            nqp = int(cnt.sum())
+            if nqp == 0:
+                continue
+            fl = float(nqp) * (K * pp + fh)
+            self.launches.append(Launch(gb, gk, b, phi.astype(np.int32), oo, nqp, fl, nqp * nc, cls,
+                                        None if plo is None else plo.astype(np.int32), prec, self.nchunk))
+            self.total_flops += fl
+        self.nchunk += 1
+        self.total_quartets += nq
+        self.total_bytes += 8.0 * nq * nc
+
+    def _check_plan(self):
+        """Every launch's output stays inside the chunk buffer (a launch with
+        a region past it would write out of bounds on the device)."""
+        for ln in self.launches:
+            nc = ln.nout // max(ln.nq, 1)
+            lo = np.zeros_like(ln.nket) if ln.lo is None else ln.lo
+            cnt = ln.nket.astype(np.int64) - lo
+            if len(cnt) and ((ln.off + cnt * nc).max() > self.out_doubles or cnt.min() < 0):
+                raise RuntimeError("launch plan exceeds the output buffer (class %s)" % (ln.cls,))
 
     def chunk_launches(self, chunk):
         """Launches writing output chunk `chunk` (index built on first use)."""

@@ -150,8 +150,13 @@ class Server:
         if deriv:
             work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
         s = torch.zeros(2, dtype=torch.float64, device="cuda")
+        t_note = time.perf_counter()
         with torch.cuda.stream(stream):
             for c in range(ss.nchunk):
+                if self.verbose and time.perf_counter() - t_note > 30.0:
+                    print("[serve %.1fs] request %d: chunk %d of %d" % (self.now(), r.rid, c, ss.nchunk),
+                          file=sys.stderr, flush=True)
+                    t_note = time.perf_counter()
                 n = ss.chunk_doubles(c)
                 ss.run_pass(out.data_ptr(), chunk=c, work_ptr=None if work is None else work.data_ptr())
                 x = out[:n]

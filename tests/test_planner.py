@@ -1,0 +1,69 @@
+"""The screened-set launch planner (CPU): every screened quartet planned
+exactly once, every launch's output inside the chunk buffer -- including a
+bra whose own kets exceed the buffer (cut into ket ranges) -- and the
+mixed-precision split covering each range without overlap."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import quartets
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INFO = os.path.join(ROOT, "simint-generator_amd", "csrc", "gen", "classes.json")
+
+
+class _Group:
+    def __init__(self, cls, nprim, bounds):
+        self.cls, self.nprim = cls, nprim
+        self.bounds = np.sort(np.asarray(bounds, dtype=np.float64))[::-1].copy()
+        self.pairs = [(i, i) for i in range(len(bounds))]
+
+
+def _planned(groups, out_doubles, mixed=None, deriv=0, tol=1e-12):
+    ss = quartets.ScreenedSet.__new__(quartets.ScreenedSet)
+    ss.tol, ss.mixed, ss.deriv, ss.out_doubles, ss.groups = tol, mixed, deriv, out_doubles, groups
+    ss.info = json.load(open(INFO))
+    ss._plan()
+    return ss
+
+
+@pytest.mark.skipif(not os.path.exists(INFO), reason="generated class table not built")
+@pytest.mark.parametrize("mixed,deriv", [(None, 0), (1e-4, 0), (None, 1)])
+def test_plan_covers_each_quartet_once_within_buffer(mixed, deriv):
+    rng = np.random.default_rng(0)
+    groups = [_Group((2, 2), 1, 10 ** rng.uniform(-14, 1, 300)),
+              _Group((2, 1), 3, 10 ** rng.uniform(-14, 1, 500)),
+              _Group((1, 0), 1, 10 ** rng.uniform(-14, 1, 200))]
+    cap = 40000 if not deriv else 300000      # (dd|dd): 1296 doubles, x12 for derivatives
+    ss = _planned(groups, cap, mixed, deriv)
+    assert any(ln.lo is not None for ln in ss.launches)          # ranges were needed
+    seen = {}
+    for ln in ss.launches:
+        nc = ln.nout // ln.nq
+        lo = np.zeros_like(ln.nket) if ln.lo is None else ln.lo
+        assert (ln.off + (ln.nket - lo) * nc).max() <= cap
+        for i in range(len(ln.nket)):
+            for k in range(int(lo[i]), int(ln.nket[i])):
+                key = (id(ln.bra), id(ln.ket), ln.b0 + i, k)
+                assert key not in seen
+                seen[key] = (ln.chunk, int(ln.off[i]) + (k - int(lo[i])) * nc)
+    # the screened set itself
+    want = 0
+    for gi, gb in enumerate(groups):
+        for gk in groups[:gi + 1]:
+            for b, vb in enumerate(gb.bounds):
+                n = int((vb * gk.bounds >= 1e-24).sum())
+                want += min(n, b + 1) if gk is gb else n
+    assert len(seen) == want == ss.total_quartets
+    # no two quartets share an output slot within a chunk
+    slots = set(seen.values())
+    assert len(slots) == len(seen)
+
+
+@pytest.mark.skipif(not os.path.exists(INFO), reason="generated class table not built")
+def test_plan_rejects_a_quartet_larger_than_the_buffer():
+    groups = [_Group((2, 2), 1, [1.0, 0.5])]
+    with pytest.raises(RuntimeError):
+        _planned(groups, 1000, deriv=1)
