@@ -229,7 +229,8 @@ def mixed_leg(natom=64, threshold=1e-4):
     ss.launches = planm
     ss.free()
     return {"workload": "%d-atom grid of the 64-atom recipe (%d shells), Schwarz 1e-12; quartets with "
-                        "sqrt((ab|ab)(cd|cd)) < %g run the FP32 VRR/HRR" % (natom, len(shells), threshold),
+                        "sqrt((ab|ab)(cd|cd)) < %g (scaled per class by its HRR sides) run the FP32 VRR"
+                        % (natom, len(shells), threshold),
             "quartets": ss.total_quartets, "fp32_quartets": nmixed,
             "mixed_value": ss.total_quartets / wm, "fp64_value": ss.total_quartets / w64,
             "speedup": w64 / wm, "mixed_device_s": dm, "fp64_device_s": d64, "unit": "shell quartets/s",
