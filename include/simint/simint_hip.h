@@ -106,6 +106,9 @@ const char *simint_hip_last_error(void);
  * negative value if that family is not built for the class. */
 int simint_hip_class_kernel(int la, int lb, int lc, int ld);
 int simint_hip_set_class_kernel(int la, int lb, int lc, int ld, int family);
+/* The same for the quartet-list calls: 1 = lane-per-quartet, 2 = wave-per-quartet. */
+int simint_hip_class_list_kernel(int la, int lb, int lc, int ld);
+int simint_hip_set_class_list_kernel(int la, int lb, int lc, int ld, int family);
 
 #ifdef __cplusplus
 }

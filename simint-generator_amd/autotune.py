@@ -24,7 +24,13 @@ def main():
     ap.add_argument("--natom", type=int, default=16)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
+    ap.add_argument("--list", action="store_true",
+                    help="tune the quartet-list kernels on the batched all-classes stream instead "
+                         "(merges 'list_family' into the output file)")
+    ap.add_argument("--nquartet", type=int, default=200000)
     a = ap.parse_args()
+    if a.list:
+        return tune_lists(a)
     import torch
     import quartets
     import simint
@@ -63,9 +69,58 @@ def main():
             family[name] = min(res, key=res.get)
             timings[name] = res
             print("%-8s %s" % (name, "  ".join("%s %.3f ms" % kv for kv in sorted(res.items()))), flush=True)
-    doc = {"workload": "config-2 recipe, %d atoms, Schwarz 1e-12, quartets of each class" % a.natom,
-           "family": family, "timings_ms": timings,
-           "generated": time.strftime("%Y-%m-%d %H:%M:%S")}
+    doc = json.load(open(a.out)) if os.path.exists(a.out) else {}
+    doc.update({"workload": "64-atom recipe, %d atoms, Schwarz 1e-12, quartets of each class" % a.natom,
+                "family": family, "timings_ms": timings,
+                "generated": time.strftime("%Y-%m-%d %H:%M:%S")})
+    with open(a.out, "w") as f:
+        json.dump(doc, f, indent=1, sort_keys=True)
+    print("wrote", a.out)
+
+
+def tune_lists(a):
+    """Quartet-list family per class, timed on a batched stream of the
+    all-classes recipe (a 4096-shell pool, depths 1-6)."""
+    import torch
+    import batch
+    import simint
+    import workloads
+    L = simint.lib()
+    shells, q = workloads.config3(nquartet=a.nquartet, seed=1)
+    basis = simint.Basis(shells)
+    b = batch.QuartetBatch(basis, q)
+    out = torch.empty(b.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(L.simint_hip_stream())
+    family, timings = {}, {}
+    for g in b.groups:
+        name = "_".join("spdfghik"[x] for x in g.cls)
+        prev = L.simint_hip_class_list_kernel(*g.cls)
+        res = {}
+        for fam, fname in ((1, "lpq"), (2, "wpq")):
+            if L.simint_hip_set_class_list_kernel(*g.cls, fam) < 0:
+                continue
+            best = None
+            for rep in range(a.reps + 1):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                simint.compute_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(),
+                                    out.data_ptr())
+                e1.record(stream)
+                simint.synchronize()
+                ms = e0.elapsed_time(e1)
+                if rep > 0:
+                    best = ms if best is None else min(best, ms)
+            res[fname] = best
+        L.simint_hip_set_class_list_kernel(*g.cls, prev)
+        if res:
+            family[name] = min(res, key=res.get)
+            timings[name] = res
+            print("%-8s %s" % (name, "  ".join("%s %.3f ms" % kv for kv in sorted(res.items()))), flush=True)
+    doc = json.load(open(a.out)) if os.path.exists(a.out) else {}
+    doc["list_family"] = family
+    doc["list_timings_ms"] = timings
+    doc["list_workload"] = "%d quartets of the all-classes stream recipe (seed 1)" % a.nquartet
     with open(a.out, "w") as f:
         json.dump(doc, f, indent=1, sort_keys=True)
     print("wrote", a.out)

@@ -57,7 +57,8 @@ EXPORTS = [
     "simint_hip_compute_eri_prefix", "simint_hip_compute_eri_diag", "simint_hip_synchronize",
     "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
     "simint_hip_compute_eri_list", "simint_hip_compute_eri_range", "simint_hip_deriv1_worksize",
-    "simint_hip_compute_eri_deriv1_range", "simint_hip_set_stream", "simint_hip_compute_eri_deriv1_list",
+    "simint_hip_compute_eri_deriv1_range", "simint_hip_set_stream", "simint_hip_class_list_kernel",
+    "simint_hip_set_class_list_kernel", "simint_hip_compute_eri_deriv1_list",
     "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
 ]
 
@@ -120,6 +121,8 @@ def lib():
         L.simint_hip_last_error.restype = ctypes.c_char_p
         L.simint_hip_class_kernel.argtypes = [c_int] * 4
         L.simint_hip_set_class_kernel.argtypes = [c_int] * 5
+        L.simint_hip_class_list_kernel.argtypes = [c_int] * 4
+        L.simint_hip_set_class_list_kernel.argtypes = [c_int] * 5
         _LIB = L
     return _LIB
 
@@ -253,9 +256,14 @@ def apply_tuning(path):
     writes it) at run time; returns the number of classes changed."""
     import json
     n = 0
-    for name, fam in json.load(open(path)).get("family", {}).items():
+    d = json.load(open(path))
+    for name, fam in d.get("family", {}).items():
         am = ["spdfghik".index(c) for c in name.split("_")]
         if lib().simint_hip_set_class_kernel(*am, FAMILIES[fam]) >= 0:
+            n += 1
+    for name, fam in d.get("list_family", {}).items():
+        am = ["spdfghik".index(c) for c in name.split("_")]
+        if lib().simint_hip_set_class_list_kernel(*am, FAMILIES[fam]) >= 0:
             n += 1
     return n
 

@@ -81,8 +81,8 @@ def test_list_both_families(gpu, family):
     L = simint.lib()
     prev = {}
     for am in itertools.product(range(4), repeat=4):
-        prev[am] = L.simint_hip_class_kernel(*am)
-        L.simint_hip_set_class_kernel(*am, family)
+        prev[am] = L.simint_hip_class_list_kernel(*am)
+        L.simint_hip_set_class_list_kernel(*am, family)
     try:
         b = batch.QuartetBatch(basis, qs)
         out = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
@@ -91,7 +91,7 @@ def test_list_both_families(gpu, family):
         got = out.cpu().numpy()
     finally:
         for am, f in prev.items():
-            L.simint_hip_set_class_kernel(*am, f)
+            L.simint_hip_set_class_list_kernel(*am, f)
     ql = [tuple(x) for x in qs]
     ref, offs, sizes = oracle.eri(sa, ql)
     ex, _, _ = oracle.md_eri(sa, ql, reference_boys=True)
