@@ -69,7 +69,7 @@ def kernel_symbol(info, cls):
     if fam == 1:
         return "eri_lpq_%s" % name
     if fam in (3, 4):
-        return "eri_lpqs%d_%s" % (info[name]["split_groups"][fam - 3], name)
+        return "eri_" + info[name]["split_kernels"][fam - 3]
     return "eri_wpq_%s" % name
 
 

@@ -116,7 +116,8 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
     # drop stale generated kernels of classes outside this build
     from emitter.hip_lpq import class_name
     want = set("eri_lpq_%s.hip" % class_name(a) for a in lpq) | set("eri_wpq_%s.hip" % class_name(a) for a in wpq)
-    want |= set("eri_lpqs%d_%s.hip" % (G, class_name(a)) for a, Gs in lpqs.items() for G in Gs)
+    from emitter.hip_lpqs import kernel_name
+    want |= set("eri_%s.hip" % kernel_name(a, G) for a, Gs in lpqs.items() for G in Gs)
     srcs = [s for s in srcs if os.path.basename(s) in want]
     rt = [os.path.join(CSRC, "runtime.hip"), os.path.join(CSRC, "shell.cpp"), os.path.join(CSRC, "shell_screen.cpp")]
     reports = {}

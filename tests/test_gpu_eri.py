@@ -348,3 +348,26 @@ def test_split_kernels_at_scale(gpu):
             assert np.abs(v - res[2]).max() <= 1e-13 * max(1.0, np.abs(res[2]).max()), (cls, fam)
         dev.free()
         tmp.free()
+
+
+def test_split_kernels_both_orientations(gpu):
+    """The split kernels against the long-double values for classes whose
+    bra outweighs the ket (bra-top groups, ket HRR first) and the reverse
+    (ket-top groups, bra HRR first), each family that is built."""
+    shells = _f_shells(seed=13)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    L = simint.lib()
+    checked = 0
+    for am in [(2, 2, 1, 1), (1, 1, 2, 2), (2, 1, 2, 2), (3, 2, 2, 1), (1, 2, 2, 3), (1, 1, 3, 3), (2, 2, 2, 2)]:
+        prev = L.simint_hip_class_kernel(*am)
+        for fam in (3, 4):
+            if L.simint_hip_set_class_kernel(*am, fam) < 0:
+                continue
+            _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]], exact=True)
+            checked += 1
+        L.simint_hip_set_class_kernel(*am, prev)
+    assert checked >= 7
