@@ -238,6 +238,42 @@ def mixed_leg(natom=64, threshold=1e-4):
             "max_abs_err_per_class": errs, "bound": 1e-10, "setup_s": setup}
 
 
+def deriv_leg(natom=64, tol=1e-12):
+    """First derivatives of every screened quartet of the 64-atom system
+    (12 values per Cartesian), one pass through a 1 GiB chunk buffer on the
+    device (simint_hip_compute_eri_deriv1_range per launch)."""
+    import torch
+    import quartets
+    import simint
+    import workloads
+    shells = workloads.config2(natom=natom, seed=0)
+    ss = quartets.ScreenedSet(shells, tol=tol, out_doubles=1 << 27, deriv=1)
+    work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
+    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
+    ss.run_pass(out.data_ptr(), chunk=0, work_ptr=work.data_ptr())
+    simint.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t = time.perf_counter()
+    e0.record(stream)
+    t_note = t
+    for c in range(ss.nchunk):
+        ss.run_pass(out.data_ptr(), chunk=c, work_ptr=work.data_ptr())
+        if time.perf_counter() - t_note > 20.0:
+            progress("deriv: chunk %d of %d" % (c, ss.nchunk))
+            t_note = time.perf_counter()
+    e1.record(stream)
+    simint.synchronize()
+    wall = time.perf_counter() - t
+    res = {"workload": "first derivatives of the %d-atom system, Schwarz %g" % (natom, tol),
+           "quartets": ss.total_quartets, "value": ss.total_quartets / wall, "unit": "shell quartets/s",
+           "pass_s": wall, "device_s": e0.elapsed_time(e1) / 1000.0,
+           "output_GBps": ss.total_bytes / wall / 1e9}
+    ss.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,7 +282,8 @@ def main():
     ap.add_argument("--natom", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched all-classes stream")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the 256-atom mixed-precision leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-deriv", action="store_true", help="skip the first-derivative leg")
     ap.add_argument("--mixed-natom", type=int, default=64,
                     help="atoms of the mixed-precision leg (the 256-atom system takes ~7 minutes)")
     ap.add_argument("--tuning", default=None, help="apply this kernel-family tuning file before running")
@@ -394,6 +431,9 @@ def main():
     if not args.no_mixed and not args.only_class:
         line["mixed"] = mixed_leg(args.mixed_natom)
         progress("mixed leg done")
+    if not args.no_deriv and not args.only_class:
+        line["deriv1"] = deriv_leg()
+        progress("deriv leg done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ss)
         progress("cpu baseline done")

@@ -33,11 +33,21 @@ const double kBoysGrid[366 * 32] = {
 
 inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
+// first-derivative assembly table of one class: for each Cartesian k of
+// the class and each explicit centre slot j (the three centres other than
+// the skipped one, in A..D order) and direction x, the offsets of the
+// (a+1) and (a-1) values in their class blocks and the factor a_x
+struct D1Entry {
+    int ip, im, ex, pad;
+};
+constexpr int D1_N = SIMINT_OSTEI_DERIV1_MAXAM + 1;
+
 struct DeviceState {
     std::once_flag once;
     bool ok = false;
     int id = 0;
     double *boys = nullptr;
+    const D1Entry *d1tab[D1_N][D1_N][D1_N][D1_N] = {};
 };
 DeviceState g_dev;
 
@@ -69,6 +79,64 @@ void device_init()
     if (e != hipSuccess) { set_err("hipMalloc(boys)", e); return; }
     e = hipMemcpy(d, kBoysGrid, sizeof(kBoysGrid), hipMemcpyHostToDevice);
     if (e != hipSuccess) { set_err("hipMemcpy(boys)", e); return; }
+    // derivative assembly tables of every class up to the derivative maximum
+    std::vector<D1Entry> all;
+    std::vector<size_t> start;
+    auto ncart_h = [](int l) { return (l + 1) * (l + 2) / 2; };
+    auto cidx = [](int ix, int iy, int iz) {
+        const int l = ix + iy + iz;
+        return (l - ix) * (l - ix + 1) / 2 + (l - ix - iy);
+    };
+    for (int a = 0; a < D1_N; a++)
+        for (int b = 0; b < D1_N; b++)
+            for (int c = 0; c < D1_N; c++)
+                for (int dd = 0; dd < D1_N; dd++) {
+                    const int am[4] = {a, b, c, dd};
+                    const int mx = std::max(std::max(a, b), std::max(c, dd));
+                    int skip = 0;
+                    while (am[skip] != mx) skip++;
+                    const int n[4] = {ncart_h(a), ncart_h(b), ncart_h(c), ncart_h(dd)};
+                    start.push_back(all.size());
+                    for (int k = 0; k < n[0] * n[1] * n[2] * n[3]; k++) {
+                        int comp[4], r = k;
+                        for (int q = 3; q >= 0; q--) { comp[q] = r % n[q]; r /= n[q]; }
+                        for (int cen = 0; cen < 4; cen++) {
+                            if (cen == skip) continue;
+                            const int l = am[cen];
+                            int e[3] = {0, 0, 0}, idx = 0;
+                            for (int ix = l; ix >= 0; ix--)
+                                for (int iy = l - ix; iy >= 0; iy--, idx++)
+                                    if (idx == comp[cen]) { e[0] = ix; e[1] = iy; e[2] = l - ix - iy; }
+                            for (int x = 0; x < 3; x++) {
+                                int dp[4] = {n[0], n[1], n[2], n[3]}, cp[4] = {comp[0], comp[1], comp[2], comp[3]};
+                                int ep[3] = {e[0], e[1], e[2]};
+                                ep[x] += 1;
+                                dp[cen] = ncart_h(l + 1);
+                                cp[cen] = cidx(ep[0], ep[1], ep[2]);
+                                D1Entry t{((cp[0] * dp[1] + cp[1]) * dp[2] + cp[2]) * dp[3] + cp[3], 0, e[x], 0};
+                                if (e[x] > 0) {
+                                    int dm[4] = {n[0], n[1], n[2], n[3]}, cm[4] = {comp[0], comp[1], comp[2], comp[3]};
+                                    int em[3] = {e[0], e[1], e[2]};
+                                    em[x] -= 1;
+                                    dm[cen] = ncart_h(l - 1);
+                                    cm[cen] = cidx(em[0], em[1], em[2]);
+                                    t.im = ((cm[0] * dm[1] + cm[1]) * dm[2] + cm[2]) * dm[3] + cm[3];
+                                }
+                                all.push_back(t);
+                            }
+                        }
+                    }
+                }
+    D1Entry *dt = nullptr;
+    e = hipMalloc(&dt, sizeof(D1Entry) * std::max(all.size(), (size_t)1));
+    if (e != hipSuccess) { set_err("hipMalloc(deriv tables)", e); return; }
+    e = hipMemcpy(dt, all.data(), sizeof(D1Entry) * all.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { set_err("hipMemcpy(deriv tables)", e); return; }
+    int k = 0;
+    for (int a = 0; a < D1_N; a++)
+        for (int b = 0; b < D1_N; b++)
+            for (int c = 0; c < D1_N; c++)
+                for (int dd = 0; dd < D1_N; dd++) g_dev.d1tab[a][b][c][dd] = dt + start[k++];
     g_dev.id = id;
     g_dev.boys = d;
     g_dev.ok = true;
@@ -363,79 +431,96 @@ struct Deriv1Args {
     long long nquartet;
     double *out;
     const long long *qout;     // per quartet output offset, or null: contiguous
+    // filled by launch_deriv1_assemble
+    long long nk = 0;          // Cartesians of the class
+    const D1Entry *tab = nullptr;
+    int cen[3] = {0, 0, 0};    // explicit centres in slot order
+    long long np[3] = {0, 0, 0}, nm[3] = {0, 0, 0};   // (a+1), (a-1) block sizes per slot
+    const double *ps[3] = {nullptr, nullptr, nullptr};   // plus / minus per slot (the kernel
+    const double *ms[3] = {nullptr, nullptr, nullptr};   // indexes these by the unrolled slot only)
 };
 
-__device__ inline int dev_ncart(int l) { return (l + 1) * (l + 2) / 2; }
-__device__ inline int dev_cart_index(int ix, int iy, int iz)
+__global__ void __launch_bounds__(256) deriv1_assemble(const Deriv1Args d)
 {
-    const int l = ix + iy + iz;
-    return (l - ix) * (l - ix + 1) / 2 + (l - ix - iy);
+    // the 12 values of the block's 256 Cartesians are gathered in LDS and
+    // written as one contiguous run when the output is dense
+    __shared__ double stage[256 * 12];
+    const long long total = d.nquartet * d.nk;
+    const long long gid0 = (long long)blockIdx.x * 256;
+    const long long gid = gid0 + threadIdx.x;
+    const bool live = gid < total;
+    double v12[12];
+    if (live) {
+        const long long q = gid / d.nk;
+        const int k = (int)(gid - q * d.nk);
+        const D1Entry *__restrict__ t = d.tab + (long long)k * 9;
+        double s3[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double *__restrict__ P = d.ps[j] + q * d.np[j];
+            const double *__restrict__ M = d.ms[j] ? d.ms[j] + q * d.nm[j] : P;
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                const D1Entry e = t[j * 3 + x];
+                double v = P[e.ip];
+                if (e.ex > 0) v -= (double)e.ex * M[e.im];
+                v12[j * 3 + x] = v;
+                s3[x] -= v;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 3; x++) v12[9 + x] = s3[x];
+    }
+    if (d.qout) {
+        if (!live) return;
+        const long long q = gid / d.nk;
+        const int k = (int)(gid - q * d.nk);
+        double *__restrict__ o = d.out + d.qout[q] + (long long)k * 12;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int x = 0; x < 3; x++) o[d.cen[j] * 3 + x] = v12[j * 3 + x];
+#pragma unroll
+        for (int x = 0; x < 3; x++) o[d.skip * 3 + x] = v12[9 + x];
+        return;
+    }
+    if (live) {
+        double *row = stage + threadIdx.x * 12;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int x = 0; x < 3; x++) row[d.cen[j] * 3 + x] = v12[j * 3 + x];
+#pragma unroll
+        for (int x = 0; x < 3; x++) row[d.skip * 3 + x] = v12[9 + x];
+    }
+    __syncthreads();
+    const long long nval = (total - gid0 < 256 ? total - gid0 : 256) * 12;
+    double *__restrict__ o = d.out + gid0 * 12;
+    for (int i = threadIdx.x; i < nval; i += 256) o[i] = stage[i];
 }
 
-__global__ void deriv1_assemble(const Deriv1Args d)
+hipError_t launch_deriv1_assemble(Deriv1Args d, hipStream_t s)
 {
-    const int n[4] = {dev_ncart(d.am[0]), dev_ncart(d.am[1]), dev_ncart(d.am[2]), dev_ncart(d.am[3])};
-    const long long nk = (long long)n[0] * n[1] * n[2] * n[3];
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= d.nquartet * nk) return;
-    const long long q = gid / nk;
-    const int k0 = (int)(gid - q * nk);
-    int k = k0;
-    int comp[4];
-    for (int c = 3; c >= 0; c--) {
-        comp[c] = k % n[c];
-        k /= n[c];
-    }
-    double res[12];
+    if (d.am[0] >= D1_N || d.am[1] >= D1_N || d.am[2] >= D1_N || d.am[3] >= D1_N) return hipErrorInvalidValue;
+    const int n[4] = {ncart(d.am[0]), ncart(d.am[1]), ncart(d.am[2]), ncart(d.am[3])};
+    d.nk = (long long)n[0] * n[1] * n[2] * n[3];
+    d.tab = g_dev.d1tab[d.am[0]][d.am[1]][d.am[2]][d.am[3]];
+    int j = 0;
     for (int c = 0; c < 4; c++) {
         if (c == d.skip) continue;
-        // exponents of this centre's component (Psi4 order)
-        const int l = d.am[c];
-        int e[3] = {0, 0, 0};
-        {
-            int idx = 0;
-            for (int ix = l; ix >= 0; ix--)
-                for (int iy = l - ix; iy >= 0; iy--, idx++)
-                    if (idx == comp[c]) { e[0] = ix; e[1] = iy; e[2] = l - ix - iy; }
-        }
-        for (int x = 0; x < 3; x++) {
-            int ep[3] = {e[0], e[1], e[2]};
-            ep[x] += 1;
-            int dims[4] = {n[0], n[1], n[2], n[3]};
-            int cc[4] = {comp[0], comp[1], comp[2], comp[3]};
-            dims[c] = dev_ncart(l + 1);
-            cc[c] = dev_cart_index(ep[0], ep[1], ep[2]);
-            const long long np_ = (long long)dims[0] * dims[1] * dims[2] * dims[3];
-            const long long ip = ((cc[0] * dims[1] + cc[1]) * dims[2] + cc[2]) * dims[3] + cc[3];
-            double v = d.plus[c][q * np_ + ip];
-            if (e[x] > 0) {
-                int em[3] = {e[0], e[1], e[2]};
-                em[x] -= 1;
-                dims[c] = dev_ncart(l - 1);
-                cc[c] = dev_cart_index(em[0], em[1], em[2]);
-                const long long nm_ = (long long)dims[0] * dims[1] * dims[2] * dims[3];
-                const long long im_ = ((cc[0] * dims[1] + cc[1]) * dims[2] + cc[2]) * dims[3] + cc[3];
-                v -= e[x] * d.minus[c][q * nm_ + im_];
-            }
-            res[3 * c + x] = v;
-        }
+        int p[4] = {n[0], n[1], n[2], n[3]}, m[4] = {n[0], n[1], n[2], n[3]};
+        p[c] = ncart(d.am[c] + 1);
+        m[c] = d.am[c] > 0 ? ncart(d.am[c] - 1) : 0;
+        d.cen[j] = c;
+        d.ps[j] = d.plus[c];
+        d.ms[j] = d.minus[c];
+        d.np[j] = (long long)p[0] * p[1] * p[2] * p[3];
+        d.nm[j] = (long long)m[0] * m[1] * m[2] * m[3];
+        j++;
     }
-    for (int x = 0; x < 3; x++) {
-        double s = 0.0;
-        for (int c = 0; c < 4; c++)
-            if (c != d.skip) s -= res[3 * c + x];
-        res[3 * d.skip + x] = s;
-    }
-    double *o = d.qout ? d.out + d.qout[q] + (long long)k0 * 12 : d.out + gid * 12;
-    for (int t = 0; t < 12; t++) o[t] = res[t];
-}
-
-hipError_t launch_deriv1_assemble(const Deriv1Args &d, hipStream_t s)
-{
-    const long long nk = (long long)ncart(d.am[0]) * ncart(d.am[1]) * ncart(d.am[2]) * ncart(d.am[3]);
-    const long long n = d.nquartet * nk;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(deriv1_assemble, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d);
+    const long long total = d.nquartet * d.nk;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(deriv1_assemble, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d);
     return hipGetLastError();
 }
 
