@@ -11,7 +11,10 @@ time, cooperating on each:
 2. VRR in LDS: the class's recurrence nodes (one per needed (component,
    m)) are grouped into dependency levels; the lanes split each level, read
    the sources from LDS and write the result back (node descriptors are a
-   per-class constant table);
+   per-class constant table).  A thread loads the descriptors and then the
+   sources of all its nodes of a level before it stores any result, with
+   absent sources read from zero slots, so the level costs about one
+   load latency rather than one per node;
 3. contraction: lane l accumulates HRR inputs t = l, l+64, ... in registers;
 4. HRR with uniform straight-line code: bra HRR with one lane per ket-top
    component, then ket HRR with one lane per final bra component, staged
@@ -53,7 +56,10 @@ def plan(am):
     Descriptor (two 64-bit words):
       w0 = s0 | s2 << 16 | s4 << 32 | s6 << 48
       w1 = s7 | flags << 16 | f0 << 24 | f1 << 32 | f2 << 40 | f3 << 48
-      flags: bit0 ket step, bits1-2 direction, bit3..6 s2/s4/s6/s7 present."""
+      flags: bit0 ket step, bits1-2 direction, bit3..6 s2/s4/s6/s7 present.
+    An absent source points at the zero slots (len(order), len(order) + 1,
+    both 0.0 during the recurrence) with factor 0, so every node evaluates
+    the same expression without branches."""
     g = graph(*am)
     lev = levels(g)
     comps = {}
@@ -79,20 +85,22 @@ def plan(am):
             starts.append(i)
             cur += 1
     starts.append(len(order))
+    zero = len(order)
+    assert zero + 2 < 65536
     desc = []
     for n in order:
         if n.step is None:
             desc.append((0, 0))
             continue
-        sl = [slot[x] if x is not None else 0 for x in n.src]
+        sl = [slot[x] if x is not None else zero for x in n.src]
         for a, b in ((0, 1), (2, 3), (4, 5)):
             if n.src[a] is not None:
                 assert sl[b] == sl[a] + 1
         has = [n.src[2] is not None, n.src[4] is not None, n.src[6] is not None, n.src[7] is not None]
         flags = (1 if n.step in (K, L) else 0) | (n.dir << 1) | (has[0] << 3) | (has[1] << 4) \
             | (has[2] << 5) | (has[3] << 6)
-        f = n.fac
-        assert max(f) < 256
+        f = [x if h else 0 for x, h in zip(n.fac, has)]
+        assert 0 <= min(f) and max(f) < 256
         w0 = sl[0] | (sl[2] << 16) | (sl[4] << 32) | (sl[6] << 48)
         w1 = sl[7] | (flags << 16) | (f[0] << 24) | (f[1] << 32) | (f[2] << 40) | (f[3] << 48)
         desc.append((w0, w1))
@@ -111,7 +119,7 @@ def batch_size(am):
     per-workgroup budget (so 2-4 workgroups share a CU)."""
     g, order, starts, desc, tops = plan(am)
     la, lb, lc, ld = am
-    ws = max(len(order), len(tops) + ncart(la) * ncart(lb) * len(g.ket_tops()))
+    ws = max(len(order) + 2, len(tops) + ncart(la) * ncart(lb) * len(g.ket_tops()))
     npc = NPC_FIXED + g.Ltot + 1
     B = 16
     while B > 1:
@@ -123,7 +131,25 @@ def batch_size(am):
     return B, ws, npc
 
 
-PB_SETUP = 4   # primitive quartets per setup batch
+PB_SETUP = 4   # primitive quartets per setup batch (sizing B)
+
+
+LDS_CU = 160 * 1024
+
+
+def setup_batch(B, ws, npc):
+    """Primitive quartets prepared per setup batch: up to 32 (one per thread
+    row, so the Boys evaluations of a batch run side by side), as long as
+    the larger LDS image keeps as many workgroups per CU as PB_SETUP does."""
+    base = 8 * (ws * B + PB_SETUP * npc * B)
+    for PB in (32, 16, 8):
+        lds = 8 * (ws * B + PB * npc * B)
+        if PB <= THREADS // B and lds <= LDS_MAX and LDS_CU // lds == LDS_CU // base:
+            return PB
+    return PB_SETUP
+
+
+LEVEL_CHUNK = 4   # nodes per thread whose loads are issued together
 
 
 def emit_wpq(am):
@@ -149,7 +175,8 @@ def emit_wpq(am):
     w("namespace {")
     w("constexpr int NNODE = %d, NTOP = %d, NPC = %d, NE = %d, NF = %d, NAB = %d, NCD = %d;"
       % (nnode, ntop, npc, ne, nf, nab, ncd))
-    w("constexpr int B = %d, LPB = %d, NACC = %d, PB = %d, WS = %d;" % (B, LPB, nacc, PB_SETUP, ws))
+    w("constexpr int B = %d, LPB = %d, NACC = %d, PB = %d, WS = %d, ZS = %d;"
+      % (B, LPB, nacc, setup_batch(B, ws, npc), ws, nnode))
     w("constexpr int NLEV = %d;" % (len(starts) - 1))
     w("__constant__ int lev_start[%d] = {%s};" % (len(starts), ", ".join(map(str, starts))))
     w("__constant__ unsigned long long vrr_desc[%d][2] = {" % nnode)
@@ -278,35 +305,58 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
     w("            __syncthreads();")
     w("            for (int pp = 0; pp < np_batch; ++pp) {")
     w("                const double *c = pc + pp * NPC * B + b;")
-    w("                const double k6[6] = {c[0 * B], c[1 * B], c[2 * B], c[3 * B], c[4 * B], c[5 * B]};")
-    w("                const double a6[6] = {c[6 * B], c[7 * B], c[8 * B], c[9 * B], c[10 * B], c[11 * B]};")
     w("                const double aover[2] = {c[12 * B], c[13 * B]};")
     w("                const double o2[2] = {c[14 * B], c[15 * B]};")
     w("                const double o2pq = c[16 * B];")
     w("                for (int m = r0; m <= %d; m += LPB) work[m * B + b] = c[(%d + m) * B];" % (Ltot, NPC_FIXED))
+    w("                if (r0 == 0) { work[ZS * B + b] = 0.0; work[(ZS + 1) * B + b] = 0.0; }")
     w("                __syncthreads();")
+    # Each thread's nodes of a level are independent: the descriptors, then
+    # the sources, of up to LEVEL_CHUNK of them are loaded before any result
+    # is stored, so their load latencies overlap instead of adding up.  The
+    # thread row goes through an empty asm so the descriptor loads stay in
+    # the loop (hoisted, every level's descriptors would be live at once).
+    w("                int rr = r0;")
+    w('                asm volatile("" : "+v"(rr));')
     for lv in range(1, len(starts) - 1):
         S, E = starts[lv], starts[lv + 1]
-        w("                // level %d: nodes [%d, %d)" % (lv, S, E))
-        w("#pragma unroll 1")
-        w("                for (int k = %d + r0; k < %d; k += LPB) {" % (S, E))
-        w("                    {")
-        w("                        const unsigned long long w0 = vrr_desc[k][0], w1 = vrr_desc[k][1];")
-        w("                        const int fl = (int)((w1 >> 16) & 0xff);")
-        w("                        const int isket = fl & 1, sel = isket * 3 + ((fl >> 1) & 3);")
-        w("                        const double *s0 = work + (w0 & 0xffff) * B + b;")
-        w("                        double t = k6[sel] * s0[0];")
-        w("                        t = fma(a6[sel], s0[B], t);")
-        w("                        const double ao = aover[isket], oo = o2[isket];")
-        w("                        if (fl & 8) { const double *s2 = work + ((w0 >> 16) & 0xffff) * B + b;")
-        w("                            t = fma((double)((w1 >> 24) & 0xff) * oo, fma(ao, s2[B], s2[0]), t); }")
-        w("                        if (fl & 16) { const double *s4 = work + ((w0 >> 32) & 0xffff) * B + b;")
-        w("                            t = fma((double)((w1 >> 32) & 0xff) * oo, fma(ao, s4[B], s4[0]), t); }")
-        w("                        if (fl & 32) t = fma((double)((w1 >> 40) & 0xff) * o2pq, work[(w0 >> 48) * B + b], t);")
-        w("                        if (fl & 64) t = fma((double)((w1 >> 48) & 0xff) * o2pq, work[(w1 & 0xffff) * B + b], t);")
-        w("                        work[k * B + b] = t;")
-        w("                    }")
-        w("                }")
+        nj = (E - S + LPB - 1) // LPB
+        w("                // level %d: nodes [%d, %d), up to %d per thread" % (lv, S, E, nj))
+        for j0 in range(0, nj, LEVEL_CHUNK):
+            js = range(j0, min(nj, j0 + LEVEL_CHUNK))
+            w("                {")
+            for j in js:
+                w("                    const int k%d = %d + rr + %d * LPB;" % (j, S, j))
+                w("                    const int kc%d = k%d < %d ? k%d : %d;" % (j, j, E, j, E - 1))
+                w("                    const unsigned long long u%d = vrr_desc[kc%d][0], v%d = vrr_desc[kc%d][1];"
+                  % (j, j, j, j))
+            for j in js:
+                w("                    const int fl%d = (int)((v%d >> 16) & 0xff);" % (j, j))
+                w("                    const int sel%d = (fl%d & 1) * 3 + ((fl%d >> 1) & 3);" % (j, j, j))
+                w("                    const double *p0_%d = work + (u%d & 0xffff) * B + b;" % (j, j))
+                w("                    const double *p2_%d = work + ((u%d >> 16) & 0xffff) * B + b;" % (j, j))
+                w("                    const double *p4_%d = work + ((u%d >> 32) & 0xffff) * B + b;" % (j, j))
+                w("                    const double x0_%d = p0_%d[0], x1_%d = p0_%d[B];" % (j, j, j, j))
+                w("                    const double x2_%d = p2_%d[0], x3_%d = p2_%d[B];" % (j, j, j, j))
+                w("                    const double x4_%d = p4_%d[0], x5_%d = p4_%d[B];" % (j, j, j, j))
+                w("                    const double x6_%d = work[(u%d >> 48) * B + b];" % (j, j))
+                w("                    const double x7_%d = work[(v%d & 0xffff) * B + b];" % (j, j))
+                w("                    const double kk%d = c[sel%d * B], aa%d = c[(6 + sel%d) * B];" % (j, j, j, j))
+            for j in js:
+                w("                    {")
+                w("                        const int isk = fl%d & 1;" % j)
+                w("                        const double ao = aover[isk], oo = o2[isk];")
+                w("                        double t = kk%d * x0_%d;" % (j, j))
+                w("                        t = fma(aa%d, x1_%d, t);" % (j, j))
+                w("                        t = fma((double)((v%d >> 24) & 0xff) * oo, fma(ao, x3_%d, x2_%d), t);"
+                  % (j, j, j))
+                w("                        t = fma((double)((v%d >> 32) & 0xff) * oo, fma(ao, x5_%d, x4_%d), t);"
+                  % (j, j, j))
+                w("                        t = fma((double)((v%d >> 40) & 0xff) * o2pq, x6_%d, t);" % (j, j))
+                w("                        t = fma((double)((v%d >> 48) & 0xff) * o2pq, x7_%d, t);" % (j, j))
+                w("                        if (k%d < %d) work[k%d * B + b] = t;" % (j, E, j))
+                w("                    }")
+            w("                }")
         w("                __syncthreads();")
     w("#pragma unroll")
     w("                for (int s = 0; s < NACC; ++s) {")
