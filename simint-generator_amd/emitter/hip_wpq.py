@@ -311,50 +311,63 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
     w("                for (int m = r0; m <= %d; m += LPB) work[m * B + b] = c[(%d + m) * B];" % (Ltot, NPC_FIXED))
     w("                if (r0 == 0) { work[ZS * B + b] = 0.0; work[(ZS + 1) * B + b] = 0.0; }")
     w("                __syncthreads();")
-    # Each thread's nodes of a level are independent: the descriptors, then
-    # the sources, of up to LEVEL_CHUNK of them are loaded before any result
-    # is stored, so their load latencies overlap instead of adding up.  The
-    # thread row goes through an empty asm so the descriptor loads stay in
-    # the loop (hoisted, every level's descriptors would be live at once).
+    # Each thread's nodes of a level are independent: the sources of up to
+    # LEVEL_CHUNK of them are loaded before any result is stored, so their
+    # LDS latencies overlap.  The descriptors of a level (they do not depend
+    # on LDS) are loaded while the level before it runs.  The thread row goes
+    # through an empty asm so the descriptor loads stay in the loop (hoisted,
+    # every level's descriptors would be live at once).
     w("                int rr = r0;")
     w('                asm volatile("" : "+v"(rr));')
-    for lv in range(1, len(starts) - 1):
+    nlev = len(starts) - 1
+
+    def load_desc(lv):
+        S, E = starts[lv], starts[lv + 1]
+        nj = (E - S + LPB - 1) // LPB
+        for j in range(nj):
+            w("                const int k%d_%d = %d + rr + %d * LPB;" % (lv, j, S, j))
+            w("                const int kc%d_%d = k%d_%d < %d ? k%d_%d : %d;" % (lv, j, lv, j, E, lv, j, E - 1))
+            w("                const unsigned long long u%d_%d = vrr_desc[kc%d_%d][0], v%d_%d = vrr_desc[kc%d_%d][1];"
+              % (lv, j, lv, j, lv, j, lv, j))
+
+    if nlev > 1:
+        load_desc(1)
+    for lv in range(1, nlev):
         S, E = starts[lv], starts[lv + 1]
         nj = (E - S + LPB - 1) // LPB
         w("                // level %d: nodes [%d, %d), up to %d per thread" % (lv, S, E, nj))
+        if lv + 1 < nlev:
+            load_desc(lv + 1)
         for j0 in range(0, nj, LEVEL_CHUNK):
             js = range(j0, min(nj, j0 + LEVEL_CHUNK))
             w("                {")
             for j in js:
-                w("                    const int k%d = %d + rr + %d * LPB;" % (j, S, j))
-                w("                    const int kc%d = k%d < %d ? k%d : %d;" % (j, j, E, j, E - 1))
-                w("                    const unsigned long long u%d = vrr_desc[kc%d][0], v%d = vrr_desc[kc%d][1];"
-                  % (j, j, j, j))
-            for j in js:
-                w("                    const int fl%d = (int)((v%d >> 16) & 0xff);" % (j, j))
+                u, v = "u%d_%d" % (lv, j), "v%d_%d" % (lv, j)
+                w("                    const int fl%d = (int)((%s >> 16) & 0xff);" % (j, v))
                 w("                    const int sel%d = (fl%d & 1) * 3 + ((fl%d >> 1) & 3);" % (j, j, j))
-                w("                    const double *p0_%d = work + (u%d & 0xffff) * B + b;" % (j, j))
-                w("                    const double *p2_%d = work + ((u%d >> 16) & 0xffff) * B + b;" % (j, j))
-                w("                    const double *p4_%d = work + ((u%d >> 32) & 0xffff) * B + b;" % (j, j))
+                w("                    const double *p0_%d = work + (%s & 0xffff) * B + b;" % (j, u))
+                w("                    const double *p2_%d = work + ((%s >> 16) & 0xffff) * B + b;" % (j, u))
+                w("                    const double *p4_%d = work + ((%s >> 32) & 0xffff) * B + b;" % (j, u))
                 w("                    const double x0_%d = p0_%d[0], x1_%d = p0_%d[B];" % (j, j, j, j))
                 w("                    const double x2_%d = p2_%d[0], x3_%d = p2_%d[B];" % (j, j, j, j))
                 w("                    const double x4_%d = p4_%d[0], x5_%d = p4_%d[B];" % (j, j, j, j))
-                w("                    const double x6_%d = work[(u%d >> 48) * B + b];" % (j, j))
-                w("                    const double x7_%d = work[(v%d & 0xffff) * B + b];" % (j, j))
+                w("                    const double x6_%d = work[(%s >> 48) * B + b];" % (j, u))
+                w("                    const double x7_%d = work[(%s & 0xffff) * B + b];" % (j, v))
                 w("                    const double kk%d = c[sel%d * B], aa%d = c[(6 + sel%d) * B];" % (j, j, j, j))
             for j in js:
+                v = "v%d_%d" % (lv, j)
                 w("                    {")
                 w("                        const int isk = fl%d & 1;" % j)
                 w("                        const double ao = aover[isk], oo = o2[isk];")
                 w("                        double t = kk%d * x0_%d;" % (j, j))
                 w("                        t = fma(aa%d, x1_%d, t);" % (j, j))
-                w("                        t = fma((double)((v%d >> 24) & 0xff) * oo, fma(ao, x3_%d, x2_%d), t);"
-                  % (j, j, j))
-                w("                        t = fma((double)((v%d >> 32) & 0xff) * oo, fma(ao, x5_%d, x4_%d), t);"
-                  % (j, j, j))
-                w("                        t = fma((double)((v%d >> 40) & 0xff) * o2pq, x6_%d, t);" % (j, j))
-                w("                        t = fma((double)((v%d >> 48) & 0xff) * o2pq, x7_%d, t);" % (j, j))
-                w("                        if (k%d < %d) work[k%d * B + b] = t;" % (j, E, j))
+                w("                        t = fma((double)((%s >> 24) & 0xff) * oo, fma(ao, x3_%d, x2_%d), t);"
+                  % (v, j, j))
+                w("                        t = fma((double)((%s >> 32) & 0xff) * oo, fma(ao, x5_%d, x4_%d), t);"
+                  % (v, j, j))
+                w("                        t = fma((double)((%s >> 40) & 0xff) * o2pq, x6_%d, t);" % (v, j))
+                w("                        t = fma((double)((%s >> 48) & 0xff) * o2pq, x7_%d, t);" % (v, j))
+                w("                        if (k%d_%d < %d) work[k%d_%d * B + b] = t;" % (lv, j, E, lv, j))
                 w("                    }")
             w("                }")
         w("                __syncthreads();")
