@@ -27,11 +27,40 @@ OSTEI_HRR_Writer.cpp:22-99).
 from .rrgraph import graph, carts, ncart, I, J, K, L
 from .hip_lpq import class_name, _Writer
 
-# constants per primitive quartet kept in LDS (after F_0..F_L)
-#   0..2  bra PA or PB (xyz)     3..5  ket QC or QD (xyz)
-#   6..8  aop_PQ                 9..11 aoq_PQ
-#   12 a_over_p  13 a_over_q  14 1/2p  15 1/2q  16 1/2pq
-NPC_FIXED = 17
+# constants per primitive quartet kept in LDS (before F_0..F_L), in the
+# blocks a recurrence step reads together: for direction s (0..2 bra x, y,
+# z; 3..5 ket x, y, z) doubles 4s..4s+3 hold PA (or QC), aop_PQ (aoq_PQ),
+# a_over_p (-a_over_q) and 1/2p (1/2q) of that side; 24 holds 1/2pq
+NPC_FIXED = 25
+
+
+def byte_desc(desc, starts, B, LPB):
+    """Node descriptors as eight 32-bit words for a batch of B quartets:
+    LDS byte offsets (from slot 0 of quartet slot 0) of the s0, s2, s4 pairs,
+    of s6 and s7, of the node's own slot, and (from the primitive-constant
+    block) of the step direction's four constants; the four integer factors
+    as bytes.  Rows go level by level, each level padded to a multiple of
+    LPB rows (zero sources, written to the sink slot) so that every thread
+    of a level reads a row of that level.  Returns the rows and the first
+    row of each level (level 0, the Boys bases, has none)."""
+    zero, sink = len(desc), len(desc) + 2
+    z = zero * B * 8
+    pad = (z, z, z, z, z, sink * B * 8, 0, 0)
+    rows, first = [], [0]
+    for lv in range(1, len(starts) - 1):
+        first.append(len(rows))
+        S, E = starts[lv], starts[lv + 1]
+        for k in range(S, E):
+            w0, w1 = desc[k]
+            s0, s2, s4, s6 = w0 & 0xffff, (w0 >> 16) & 0xffff, (w0 >> 32) & 0xffff, w0 >> 48
+            s7 = w1 & 0xffff
+            fl = (w1 >> 16) & 0xff
+            sel = (fl & 1) * 3 + ((fl >> 1) & 3)
+            fac = [(w1 >> sh) & 0xff for sh in (24, 32, 40, 48)]
+            rows.append((s0 * B * 8, s2 * B * 8, s4 * B * 8, s6 * B * 8, s7 * B * 8, k * B * 8,
+                         4 * sel * B * 8, fac[0] | fac[1] << 8 | fac[2] << 16 | fac[3] << 24))
+        rows += [pad] * ((-(E - S)) % LPB)
+    return rows, first
 
 
 def levels(g):
@@ -119,7 +148,7 @@ def batch_size(am):
     per-workgroup budget (so 2-4 workgroups share a CU)."""
     g, order, starts, desc, tops = plan(am)
     la, lb, lc, ld = am
-    ws = max(len(order) + 2, len(tops) + ncart(la) * ncart(lb) * len(g.ket_tops()))
+    ws = max(len(order) + 3, len(tops) + ncart(la) * ncart(lb) * len(g.ket_tops()))
     npc = NPC_FIXED + g.Ltot + 1
     B = 16
     while B > 1:
@@ -179,9 +208,10 @@ def emit_wpq(am):
       % (B, LPB, nacc, setup_batch(B, ws, npc), ws, nnode))
     w("constexpr int NLEV = %d;" % (len(starts) - 1))
     w("__constant__ int lev_start[%d] = {%s};" % (len(starts), ", ".join(map(str, starts))))
-    w("__constant__ unsigned long long vrr_desc[%d][2] = {" % nnode)
-    for d in desc:
-        w("    {0x%xull, 0x%xull}," % d)
+    rows, _ = byte_desc(desc, starts, B, LPB)
+    w("__constant__ __attribute__((aligned(16))) unsigned int vrr_desc[%d][8] = {" % max(1, len(rows)))
+    for d in rows:
+        w("    {%s}," % ", ".join("%du" % x for x in d))
     w("};")
     w("__constant__ unsigned short top_slot[%d] = {%s};" % (ntop, ", ".join(map(str, tops))))
     w("}  // namespace")
@@ -206,8 +236,11 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
     else:
         w("__global__ void __launch_bounds__(%d) eri_wpq_%s(const EriArgs A)" % (THREADS, name))
     w("{")
-    w("    __shared__ double work[WS * B];")
-    w("    __shared__ double pc[PB * NPC * B];")
+    # one LDS array (at LDS address 0, so descriptor offsets address it
+    # directly): VRR / HRR work area, primitive constants, list depths
+    w("    __shared__ double lds[WS * B + PB * NPC * B + (B + 1) / 2];")
+    w("    double *const work = lds;")
+    w("    double *const pc = lds + WS * B;")
     bra_pa = "pa" if g.bra_j else "pb"
     ket_pa = "pa" if g.ket_l else "pb"
     if not listmode:
@@ -230,7 +263,7 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
         w("        const int ql = (q0 + b) < SIMINT_TILE ? (q0 + b) : (SIMINT_TILE - 1);")
         w("        const long long tl = (long long)wi.tile * SIMINT_TILE + ql;")
     else:
-        w("    __shared__ int kk[B];")
+        w("    int *const kk = (int *)(lds + WS * B + PB * NPC * B);")
         w("    const long long qbeg = (long long)blockIdx.x * B;")
         w("    const long long qend = (qbeg + B) < A.nquartet ? (qbeg + B) : A.nquartet;")
         w("    for (long long q0 = qbeg; q0 < qend; q0 += B) {")
@@ -291,23 +324,27 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
     w("                boys_eval<%d>(grid, R2 * alpha, F);" % Ltot)
     w("                const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
     w("                double *c = pc + r0 * NPC * B + b;")
-    w("                c[0 * B] = A.bra.%sx[i]; c[1 * B] = A.bra.%sy[i]; c[2 * B] = A.bra.%sz[i];" % (bra_pa, bra_pa, bra_pa))
-    w("                c[3 * B] = A.ket.%sx[kj]; c[4 * B] = A.ket.%sy[kj]; c[5 * B] = A.ket.%sz[kj];" % (ket_pa, ket_pa, ket_pa))
+    w("                c[0 * B] = A.bra.%sx[i]; c[4 * B] = A.bra.%sy[i]; c[8 * B] = A.bra.%sz[i];" % (bra_pa, bra_pa, bra_pa))
+    w("                c[12 * B] = A.ket.%sx[kj]; c[16 * B] = A.ket.%sy[kj]; c[20 * B] = A.ket.%sz[kj];" % (ket_pa, ket_pa, ket_pa))
     w("                const double a_over_p = (-alpha) * P_inv;")
     w("                const double a_over_q = alpha * Q_inv;")
-    w("                c[6 * B] = a_over_p * PQx; c[7 * B] = a_over_p * PQy; c[8 * B] = a_over_p * PQz;")
-    w("                c[9 * B] = a_over_q * PQx; c[10 * B] = a_over_q * PQy; c[11 * B] = a_over_q * PQz;")
-    w("                c[12 * B] = a_over_p; c[13 * B] = -a_over_q;")
-    w("                c[14 * B] = 0.5 * P_inv; c[15 * B] = 0.5 * Q_inv; c[16 * B] = 0.5 * one_over_PQalpha_sum;")
+    w("                c[1 * B] = a_over_p * PQx; c[5 * B] = a_over_p * PQy; c[9 * B] = a_over_p * PQz;")
+    w("                c[13 * B] = a_over_q * PQx; c[17 * B] = a_over_q * PQy; c[21 * B] = a_over_q * PQz;")
+    w("#pragma unroll")
+    w("                for (int s = 0; s < 3; ++s) {")
+    w("                    c[(4 * s + 2) * B] = a_over_p; c[(4 * s + 3) * B] = 0.5 * P_inv;")
+    w("                    c[(4 * s + 14) * B] = -a_over_q; c[(4 * s + 15) * B] = 0.5 * Q_inv;")
+    w("                }")
+    w("                c[24 * B] = 0.5 * one_over_PQalpha_sum;")
     w("#pragma unroll")
     w("                for (int m = 0; m <= %d; ++m) c[(%d + m) * B] = F[m] * prefac;" % (Ltot, NPC_FIXED))
     w("            }")
     w("            __syncthreads();")
     w("            for (int pp = 0; pp < np_batch; ++pp) {")
     w("                const double *c = pc + pp * NPC * B + b;")
-    w("                const double aover[2] = {c[12 * B], c[13 * B]};")
-    w("                const double o2[2] = {c[14 * B], c[15 * B]};")
-    w("                const double o2pq = c[16 * B];")
+    w("                const double o2pq = c[24 * B];")
+    w("                char *const wb = (char *)(work + b);")
+    w("                const char *cb = (const char *)c;")
     w("                for (int m = r0; m <= %d; m += LPB) work[m * B + b] = c[(%d + m) * B];" % (Ltot, NPC_FIXED))
     w("                if (r0 == 0) { work[ZS * B + b] = 0.0; work[(ZS + 1) * B + b] = 0.0; }")
     w("                __syncthreads();")
@@ -321,14 +358,21 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
     w('                asm volatile("" : "+v"(rr));')
     nlev = len(starts) - 1
 
+    w("                const unsigned rofs = (unsigned)rr * 32u;")
+
+    _, first = byte_desc(plan(am)[3], starts, B, LPB)
+
     def load_desc(lv):
         S, E = starts[lv], starts[lv + 1]
         nj = (E - S + LPB - 1) // LPB
         for j in range(nj):
-            w("                const int k%d_%d = %d + rr + %d * LPB;" % (lv, j, S, j))
-            w("                const int kc%d_%d = k%d_%d < %d ? k%d_%d : %d;" % (lv, j, lv, j, E, lv, j, E - 1))
-            w("                const unsigned long long u%d_%d = vrr_desc[kc%d_%d][0], v%d_%d = vrr_desc[kc%d_%d][1];"
+            w("                const uint4 *dp%d_%d = (const uint4 *)((const char *)vrr_desc + (rofs + %du));"
+              % (lv, j, (first[lv] + j * LPB) * 32))
+            w("                const uint4 da%d_%d = dp%d_%d[0], db%d_%d = dp%d_%d[1];"
               % (lv, j, lv, j, lv, j, lv, j))
+
+    def ld(base, off):
+        return "*(const double *)(%s + %s)" % (base, off)
 
     if nlev > 1:
         load_desc(1)
@@ -342,32 +386,32 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
             js = range(j0, min(nj, j0 + LEVEL_CHUNK))
             w("                {")
             for j in js:
-                u, v = "u%d_%d" % (lv, j), "v%d_%d" % (lv, j)
-                w("                    const int fl%d = (int)((%s >> 16) & 0xff);" % (j, v))
-                w("                    const int sel%d = (fl%d & 1) * 3 + ((fl%d >> 1) & 3);" % (j, j, j))
-                w("                    const double *p0_%d = work + (%s & 0xffff) * B + b;" % (j, u))
-                w("                    const double *p2_%d = work + ((%s >> 16) & 0xffff) * B + b;" % (j, u))
-                w("                    const double *p4_%d = work + ((%s >> 32) & 0xffff) * B + b;" % (j, u))
-                w("                    const double x0_%d = p0_%d[0], x1_%d = p0_%d[B];" % (j, j, j, j))
-                w("                    const double x2_%d = p2_%d[0], x3_%d = p2_%d[B];" % (j, j, j, j))
-                w("                    const double x4_%d = p4_%d[0], x5_%d = p4_%d[B];" % (j, j, j, j))
-                w("                    const double x6_%d = work[(%s >> 48) * B + b];" % (j, u))
-                w("                    const double x7_%d = work[(%s & 0xffff) * B + b];" % (j, v))
-                w("                    const double kk%d = c[sel%d * B], aa%d = c[(6 + sel%d) * B];" % (j, j, j, j))
+                a, d = "da%d_%d" % (lv, j), "db%d_%d" % (lv, j)
+                w("                    const double x0_%d = %s, x1_%d = %s;"
+                  % (j, ld("wb", a + ".x"), j, ld("wb", a + ".x + 8 * B")))
+                w("                    const double x2_%d = %s, x3_%d = %s;"
+                  % (j, ld("wb", a + ".y"), j, ld("wb", a + ".y + 8 * B")))
+                w("                    const double x4_%d = %s, x5_%d = %s;"
+                  % (j, ld("wb", a + ".z"), j, ld("wb", a + ".z + 8 * B")))
+                w("                    const double x6_%d = %s, x7_%d = %s;"
+                  % (j, ld("wb", a + ".w"), j, ld("wb", d + ".x")))
+                w("                    const char *cs%d = cb + %s.z;" % (j, d))
+                w("                    const double kk%d = %s, aa%d = %s;"
+                  % (j, ld("cs%d" % j, "0"), j, ld("cs%d" % j, "8 * B")))
+                w("                    const double ao%d = %s, oo%d = %s;"
+                  % (j, ld("cs%d" % j, "16 * B"), j, ld("cs%d" % j, "24 * B")))
             for j in js:
-                v = "v%d_%d" % (lv, j)
+                f = "db%d_%d.w" % (lv, j)
                 w("                    {")
-                w("                        const int isk = fl%d & 1;" % j)
-                w("                        const double ao = aover[isk], oo = o2[isk];")
                 w("                        double t = kk%d * x0_%d;" % (j, j))
                 w("                        t = fma(aa%d, x1_%d, t);" % (j, j))
-                w("                        t = fma((double)((%s >> 24) & 0xff) * oo, fma(ao, x3_%d, x2_%d), t);"
-                  % (v, j, j))
-                w("                        t = fma((double)((%s >> 32) & 0xff) * oo, fma(ao, x5_%d, x4_%d), t);"
-                  % (v, j, j))
-                w("                        t = fma((double)((%s >> 40) & 0xff) * o2pq, x6_%d, t);" % (v, j))
-                w("                        t = fma((double)((%s >> 48) & 0xff) * o2pq, x7_%d, t);" % (v, j))
-                w("                        if (k%d_%d < %d) work[k%d_%d * B + b] = t;" % (lv, j, E, lv, j))
+                w("                        t = fma((double)(%s & 0xff) * oo%d, fma(ao%d, x3_%d, x2_%d), t);"
+                  % (f, j, j, j, j))
+                w("                        t = fma((double)((%s >> 8) & 0xff) * oo%d, fma(ao%d, x5_%d, x4_%d), t);"
+                  % (f, j, j, j, j))
+                w("                        t = fma((double)((%s >> 16) & 0xff) * o2pq, x6_%d, t);" % (f, j))
+                w("                        t = fma((double)(%s >> 24) * o2pq, x7_%d, t);" % (f, j))
+                w("                        *(double *)(wb + db%d_%d.y) = t;" % (lv, j))
                 w("                    }")
             w("                }")
         w("                __syncthreads();")
