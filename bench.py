@@ -73,6 +73,13 @@ def kernel_symbol(info, cls):
     return "eri_wpq_%s" % name
 
 
+def list_kernel_symbol(cls):
+    """Name of the quartet-list kernel the class runs now."""
+    import simint
+    name = "_".join("spdfghik"[x] for x in cls)
+    return ("eri_lpql_%s" if simint.lib().simint_hip_class_list_kernel(*cls) == 1 else "eri_wpql_%s") % name
+
+
 def cpu_baseline(ss, seconds=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -159,8 +166,14 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
                      % (len(b.groups), BATCH_STREAMS),
            "setup_s": setup,
            "top_classes": [{"class": "_".join("spdfghik"[x] for x in g.cls), "ms": ms, "quartets": g.n,
-                            "kernel": kernel_symbol(info, g.cls)}
-                           for ms, g in per[:6]]}
+                            "kernel": list_kernel_symbol(g.cls)}
+                           for ms, g in per[:8]]}
+    fam = {}
+    for ms, g in per:
+        k = "wpql" if "wpql" in list_kernel_symbol(g.cls) else "lpql"
+        fam[k] = fam.get(k, 0.0) + ms
+    res["eager_class_ms"] = {"total": sum(ms for ms, _ in per), "by_family": fam,
+                             "note": "classes one after another on one stream, per-class events"}
     b.free()
     return res
 

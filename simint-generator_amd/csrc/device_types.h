@@ -72,6 +72,7 @@ struct EriListArgs {
     const double *boys_grid;
     double tol2;
     double ket_screen_max;
+    int split;                 // lanes per quartet (1, 2, 4, 8) of the lane-per-quartet list kernels
 };
 
 struct EriArgs {

@@ -1019,7 +1019,21 @@ EriListArgs list_args(const simint_hip_pairs *P, const simint_hip_pairs *Q, long
     a.boys_grid = g_dev.boys;
     a.tol2 = (screen_tol > 0.0 && P->has_screen && Q->has_screen) ? screen_tol * screen_tol : 0.0;
     a.ket_screen_max = Q->screen_max;
+    a.split = 1;
     return a;
+}
+
+thread_local int t_list_split = 0;   // 0: by list length
+
+// Lanes per quartet for a lane-per-quartet list launch of n quartets: enough
+// that the launch has about 16384 lanes (a short list of one class otherwise
+// occupies a few dozen wavefronts), at most 8.
+int list_split(long long n)
+{
+    if (t_list_split > 0) return t_list_split;
+    int s = 1;
+    while (s < 8 && n * s < 16384) s *= 2;
+    return s;
 }
 
 }  // namespace
@@ -1039,6 +1053,7 @@ long long simint_hip_compute_eri_list(const simint_hip_pairs *P, const simint_hi
     a.qout = d_qout;
     a.ostride = (long long)ncart(P->am1) * ncart(P->am2) * ncart(Q->am1) * ncart(Q->am2);
     a.out = d_out;
+    a.split = list_split(nquartet);
     hipError_t e = fn(&a, ctx().s);
     if (e != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
     return nquartet;
@@ -1217,6 +1232,14 @@ void *simint_hip_stream(void)
 {
     if (!ensure_device()) return nullptr;
     return (void *)ctx().s;
+}
+
+int simint_hip_set_list_split(int lanes)
+{
+    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return SIMINT_ERROR_ARGUMENT;
+    const int prev = t_list_split;
+    t_list_split = lanes;
+    return prev;
 }
 
 int simint_hip_set_stream(void *stream)

@@ -344,38 +344,51 @@ def _emit_lpq_prefix(am, w, fp, hrr_fp="double"):
 
 
 def emit_lpq_list(am, w):
-    """Quartet-list variant: lane l of block k takes quartet 64k + l, with
-    its own bra and ket pair (both read from the linear view, per-lane
-    gathers), its own primitive loops, and a contiguous output block."""
+    """Quartet-list variant: S = A.split lanes per quartet (a power of two
+    up to 8); lanes 64k + l take quartet (64k + l) / S, with its own bra and
+    ket pair (both read from the linear view, per-lane gathers).  Lane
+    sub = l % S sums primitive quartets sub, sub + S, ... (bra-major order,
+    as the reference's loops); the S partial sums are added across the
+    lanes (xor butterfly), then lane sub = 0 runs the HRR and writes the
+    contiguous output block.  With S = 1 the summation order is the
+    lane-per-quartet kernels'."""
     g = graph(*am)
     name = class_name(am)
     ntop = len(g.top_keys)
     w("__global__ void __launch_bounds__(64) eri_lpql_%s(const EriListArgs A)" % name)
     w("{")
-    w("    const long long q = (long long)blockIdx.x * 64 + threadIdx.x;")
-    w("    if (q >= A.nquartet) return;")
-    w("    const int bp = A.qbra[q], kp = A.qket[q];")
+    w("    const int S = A.split;")
+    w("    const long long gl = (long long)blockIdx.x * 64 + threadIdx.x;")
+    w("    const long long q = gl / S;")
+    w("    const int sub = (int)(gl - q * S);")
+    w("    const bool valid = q < A.nquartet;")
+    w("    // lanes past the list stay until the cross-lane sums")
+    w("    const long long qc = valid ? q : A.nquartet - 1;")
+    w("    const int bp = A.qbra[qc], kp = A.qket[qc];")
     w("    const double *__restrict__ grid = A.boys_grid;")
     w("    const int ib0 = A.bra.poff[bp];")
-    w("    const int ib1 = ib0 + A.bra.np[bp];")
     w("    const int jb0 = A.ket.poff[kp];")
-    w("    const int jb1 = jb0 + A.ket.np[kp];")
+    w("    const int nqk = A.ket.np[kp];")
+    w("    const int K = valid ? A.bra.np[bp] * nqk : 0;")
     w("    const bool screen = A.tol2 > 0.0;")
     for t in range(ntop):
         w("    double acc%d = 0.0;" % t)
     bra_pa = "pa" if g.bra_j else "pb"
     ket_pa = "pa" if g.ket_l else "pb"
-    w("    for (int i = ib0; i < ib1; ++i) {")
+    w("    int ii = 0, jj = sub;")
+    w("    for (int p = sub; p < K; p += S) {")
+    w("        while (jj >= nqk) { jj -= nqk; ++ii; }")
+    w("        const int i = ib0 + ii, kj = jb0 + jj;")
+    w("        jj += S;")
+    w("        const double P_screen = screen ? A.bra.screen[i] : 0.0;")
+    w("        if (screen && (P_screen * A.ket_screen_max < A.tol2 || A.ket.screen[kj] * P_screen < A.tol2)) continue;")
     w("        const double P_alpha = A.bra.alpha[i];")
     w("        const double P_inv = A.bra.inv_alpha[i];")
     w("        const double P_prefac = A.bra.prefac[i];")
     w("        const double P_x = A.bra.x[i], P_y = A.bra.y[i], P_z = A.bra.z[i];")
     w("        const double P_%s[3] = {A.bra.%sx[i], A.bra.%sy[i], A.bra.%sz[i]};"
       % ("PA" if g.bra_j else "PB", bra_pa, bra_pa, bra_pa))
-    w("        const double P_screen = screen ? A.bra.screen[i] : 0.0;")
-    w("        if (screen && P_screen * A.ket_screen_max < A.tol2) continue;")
-    w("        for (int kj = jb0; kj < jb1; ++kj) {")
-    w("            if (screen && A.ket.screen[kj] * P_screen < A.tol2) continue;")
+    w("        {")
     w("            const double Q_alpha = A.ket.alpha[kj];")
     w("            const double Q_inv = A.ket.inv_alpha[kj];")
     w("            const double Q_prefac = A.ket.prefac[kj];")
@@ -385,6 +398,11 @@ def emit_lpq_list(am, w):
     emit_body(am, w, indent="            ")
     w("        }")
     w("    }")
+    w("    for (int o = 1; o < S; o <<= 1) {")
+    for t in range(ntop):
+        w("        acc%d += __shfl_xor(acc%d, o);" % (t, t))
+    w("    }")
+    w("    if (!valid || sub != 0) return;")
     w("    double *__restrict__ o = A.out + (A.qout ? A.qout[q] : q * A.ostride);")
     if g.bra_hrr_needed():
         w("    const double hAB[3] = {A.bra.abx[bp], A.bra.aby[bp], A.bra.abz[bp]};")
@@ -396,6 +414,8 @@ def emit_lpq_list(am, w):
     w('extern "C" hipError_t simint_launch_lpql_%s(const EriListArgs *a, hipStream_t s)' % name)
     w("{")
     w("    if (a->nquartet <= 0) return hipSuccess;")
-    w("    hipLaunchKernelGGL(eri_lpql_%s, dim3((unsigned)((a->nquartet + 63) / 64)), dim3(64), 0, s, *a);" % name)
+    w("    if (a->split < 1 || a->split > 8 || (a->split & (a->split - 1))) return hipErrorInvalidValue;")
+    w("    hipLaunchKernelGGL(eri_lpql_%s, dim3((unsigned)((a->nquartet * a->split + 63) / 64)), dim3(64), 0, s, *a);"
+      % name)
     w("    return hipGetLastError();")
     w("}")

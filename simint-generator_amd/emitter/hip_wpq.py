@@ -179,6 +179,9 @@ def setup_batch(B, ws, npc):
 
 
 LEVEL_CHUNK = 4   # nodes per thread whose loads are issued together
+LEVEL_CHUNK_ONE = 8   # the same for classes with one quartet per batch:
+                      # they run at one wave per SIMD, with the register
+                      # room for twice the loads in flight
 
 
 def emit_wpq(am):
@@ -382,8 +385,9 @@ def _emit_kernel(g, w, am, name, starts, Ltot, B, LPB, listmode):
         w("                // level %d: nodes [%d, %d), up to %d per thread" % (lv, S, E, nj))
         if lv + 1 < nlev:
             load_desc(lv + 1)
-        for j0 in range(0, nj, LEVEL_CHUNK):
-            js = range(j0, min(nj, j0 + LEVEL_CHUNK))
+        chunk = LEVEL_CHUNK_ONE if B == 1 else LEVEL_CHUNK
+        for j0 in range(0, nj, chunk):
+            js = range(j0, min(nj, j0 + chunk))
             w("                {")
             for j in js:
                 a, d = "da%d_%d" % (lv, j), "db%d_%d" % (lv, j)

@@ -58,6 +58,7 @@ EXPORTS = [
     "simint_hip_stream", "simint_hip_last_error", "simint_hip_class_kernel", "simint_hip_set_class_kernel",
     "simint_hip_compute_eri_list", "simint_hip_compute_eri_range", "simint_hip_deriv1_worksize",
     "simint_hip_compute_eri_deriv1_range", "simint_hip_set_stream", "simint_hip_class_list_kernel",
+    "simint_hip_set_list_split",
     "simint_hip_set_class_list_kernel", "simint_hip_compute_eri_deriv1_list",
     "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
 ]
@@ -118,6 +119,7 @@ def lib():
         L.simint_hip_graph_free.argtypes = [c_void_p]
         L.simint_hip_stream.restype = c_void_p
         L.simint_hip_set_stream.argtypes = [c_void_p]
+        L.simint_hip_set_list_split.argtypes = [c_int]
         L.simint_hip_last_error.restype = ctypes.c_char_p
         L.simint_hip_class_kernel.argtypes = [c_int] * 4
         L.simint_hip_set_class_kernel.argtypes = [c_int] * 5

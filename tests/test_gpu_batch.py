@@ -100,6 +100,35 @@ def test_list_both_families(gpu, family):
     b.free()
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_list_lanes_per_quartet(gpu, lanes):
+    """Lane-per-quartet list kernels with 1, 2, 4 or 8 lanes per quartet
+    (each lane a strided share of the primitive quartets, shares added across
+    the lanes) against the oracle on the all-classes stream."""
+    import torch
+    shells = _pool(seed=6, per_am=4, maxk=4)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    qs = _stream(shells, per_class=1, seed=3)
+    L = simint.lib()
+    prev = L.simint_hip_set_list_split(lanes)
+    try:
+        b = batch.QuartetBatch(basis, qs)
+        out = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
+        b.run(out.data_ptr())
+        simint.synchronize()
+        got = out.cpu().numpy()
+    finally:
+        L.simint_hip_set_list_split(prev)
+    ql = [tuple(x) for x in qs]
+    ref, offs, sizes = oracle.eri(sa, ql)
+    ex, _, _ = oracle.md_eri(sa, ql, reference_boys=True)
+    ok, st = parity_report(got, ref, offs, sizes, ex)
+    print(st)
+    assert ok, st
+    b.free()
+
+
 def test_list_graph_replay(gpu):
     """The batch captured into one graph: every replay (with eager work and
     synchronisation between replays) equals the eager result bit for bit."""
