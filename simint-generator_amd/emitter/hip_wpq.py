@@ -157,6 +157,11 @@ def batch_size(am):
         B //= 2
     if 8 * (ws * B + PB_SETUP * npc * B) > LDS_MAX:
         raise ValueError("class %s needs %d bytes of LDS" % (am, 8 * (ws + PB_SETUP * npc)))
+    # a quartet taking more than 64 KB leaves room for at most one workgroup
+    # per CU alongside its register use anyway: take two quartets per batch
+    # where they fit (twice the independent nodes per level barrier)
+    if B == 1 and 8 * ws > 64 * 1024 and 8 * (2 * ws + 2 * PB_SETUP * npc) <= LDS_MAX:
+        B = 2
     return B, ws, npc
 
 
