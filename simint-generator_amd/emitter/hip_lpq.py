@@ -61,9 +61,18 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     used_steps = set(n.step for n in nodes if n.step is not None)
     has_bra = bool(used_steps & {I, J})
     has_ket = bool(used_steps & {K, L})
+    # classes without HRR take 1/(p+q) and its square root from one refined
+    # reciprocal square root (a few ulp): no HRR transfer amplifies it.  Only
+    # in the lane-per-quartet kernels (their file carries the helpers); the
+    # setup is FP64 in the mixed kernels too.
+    fast = keep is None and not g.bra_hrr_needed() and not g.ket_hrr_needed()
     w(indent + "const double PQalpha_mul = P_alpha * Q_alpha;")
     w(indent + "const double PQalpha_sum = P_alpha + Q_alpha;")
-    w(indent + "const double one_over_PQalpha_sum = 1.0 / PQalpha_sum;")
+    if fast:
+        w(indent + "const double rs_PQalpha_sum = rsqrt_nr(PQalpha_sum);")
+        w(indent + "const double one_over_PQalpha_sum = rs_PQalpha_sum * rs_PQalpha_sum;")
+    else:
+        w(indent + "const double one_over_PQalpha_sum = 1.0 / PQalpha_sum;")
     w(indent + "const double PQx = P_x - Q_x;")
     w(indent + "const double PQy = P_y - Q_y;")
     w(indent + "const double PQz = P_z - Q_z;")
@@ -73,8 +82,12 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     w(indent + "const double alpha = PQalpha_mul * one_over_PQalpha_sum;")
     w(indent + "const double F_x = R2 * alpha;")
     w(indent + "double F[%d];" % (Ltot + 1))
-    w(indent + "boys_eval<%d>(grid, F_x, F);" % Ltot)
-    w(indent + "const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
+    if fast:
+        w(indent + "boys_eval_rs<%d>(grid, F_x, F);" % Ltot)
+        w(indent + "const double prefac = (P_prefac * Q_prefac) * rs_PQalpha_sum;")
+    else:
+        w(indent + "boys_eval<%d>(grid, F_x, F);" % Ltot)
+        w(indent + "const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
     # integer constants k/(2p), k/(2q), k/(2pq)
     need = {"2p": set(), "2q": set(), "2pq": set()}
     for n in nodes:
@@ -262,6 +275,45 @@ def emit_hrr(am, w, indent="    ", out="o", acc="acc", fp="double", staged=False
                         w(indent + "%s[%d] = %s;" % (out, k, val))
 
 
+def _emit_fast_setup(w, Ltot):
+    """Helpers of the classes without HRR (emit_body's fast setup): x^-1/2
+    from the hardware reciprocal square root refined by two Newton steps
+    y <- y + y (1/2 - (x/2) y^2) (within a few ulp of the correctly rounded
+    value; IEEE 1/x and sqrt are each about three times the instructions),
+    and the Boys function with its long-range branch for L < 4 (all orders
+    from longfac[m] x^-(m+1/2), boys.h:18-25) on that square root (L >= 4
+    keeps boys_eval's branch)."""
+    w("namespace {")
+    w("__device__ __forceinline__ double rsqrt_nr(double x)")
+    w("{")
+    w("    double y = __builtin_amdgcn_rsq(x);")
+    w("    const double h = 0.5 * x;")
+    w("    double t = fma(-h * y, y, 0.5);")
+    w("    y = fma(y, t, y);")
+    w("    t = fma(-h * y, y, 0.5);")
+    w("    return fma(y, t, y);")
+    w("}")
+    w("")
+    w("template <int L>")
+    w("__device__ __forceinline__ void boys_eval_rs(const double *__restrict__ grid, double x, double *F)")
+    w("{")
+    w("    if (L >= 4 || x < SIMINT_BOYS_MAXX) {")
+    w("        boys_eval<L>(grid, x, F);")
+    w("        return;")
+    w("    }")
+    w("    constexpr double longfac[] = SIMINT_BOYS_LONGFAC;")
+    w("    double x2 = rsqrt_nr(x);")
+    w("    const double x1 = x2 * x2;")
+    w("#pragma unroll")
+    w("    for (int n = 0; n <= L; n++) {")
+    w("        F[n] = longfac[n] * x2;")
+    w("        x2 *= x1;")
+    w("    }")
+    w("}")
+    w("}  // namespace")
+    w("")
+
+
 def emit_lpq(am, mixed=False):
     """Source of one class: the prefix kernel, the quartet-list kernel and,
     with mixed=True, the FP32-VRR prefix kernel (eri_lpqm_<class>)."""
@@ -270,6 +322,9 @@ def emit_lpq(am, mixed=False):
     w('#include "../device_types.h"')
     w('#include "../boys_device.h"')
     w("")
+    g = graph(*am)
+    if not g.bra_hrr_needed() and not g.ket_hrr_needed():
+        _emit_fast_setup(w, g.Ltot)
     _emit_lpq_prefix(am, w, "double")
     emit_lpq_list(am, w)
     if mixed:
