@@ -96,11 +96,11 @@ void *simint_hip_stream(void);                /* hipStream_t of the calling thre
  * reusing those buffers: synchronise the previous stream yourself before
  * switching between such calls; list calls take device arrays and need no care. */
 int   simint_hip_set_stream(void * stream);
-/* Lanes per quartet (1, 2, 4 or 8) of the calling thread's later
+/* Lanes per quartet (1, 2, 4, 8 or 16) of the calling thread's later
  * lane-per-quartet simint_hip_compute_eri_list launches: each lane sums a
  * strided share of the quartet's primitive quartets and the shares are added
  * across the lanes before the HRR.  0 (the default): chosen by list length,
- * so that a launch has about 16384 lanes.  Returns the previous setting or
+ * so that a launch has about 65536 lanes.  Returns the previous setting or
  * a negative SIMINT_ERROR_*.  No counterpart in the reference (its kernels
  * vectorise over kets on the CPU). */
 int   simint_hip_set_list_split(int lanes);

@@ -1026,13 +1026,13 @@ EriListArgs list_args(const simint_hip_pairs *P, const simint_hip_pairs *Q, long
 thread_local int t_list_split = 0;   // 0: by list length
 
 // Lanes per quartet for a lane-per-quartet list launch of n quartets: enough
-// that the launch has about 16384 lanes (a short list of one class otherwise
-// occupies a few dozen wavefronts), at most 8.
+// that the launch has about 65536 lanes (a short list of one class otherwise
+// occupies a few dozen wavefronts), at most 16.
 int list_split(long long n)
 {
     if (t_list_split > 0) return t_list_split;
     int s = 1;
-    while (s < 8 && n * s < 16384) s *= 2;
+    while (s < 16 && n * s < 65536) s *= 2;
     return s;
 }
 
@@ -1236,7 +1236,7 @@ void *simint_hip_stream(void)
 
 int simint_hip_set_list_split(int lanes)
 {
-    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return SIMINT_ERROR_ARGUMENT;
+    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16) return SIMINT_ERROR_ARGUMENT;
     const int prev = t_list_split;
     t_list_split = lanes;
     return prev;

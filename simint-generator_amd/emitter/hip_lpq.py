@@ -400,7 +400,7 @@ def _emit_lpq_prefix(am, w, fp, hrr_fp="double"):
 
 def emit_lpq_list(am, w):
     """Quartet-list variant: S = A.split lanes per quartet (a power of two
-    up to 8); lanes 64k + l take quartet (64k + l) / S, with its own bra and
+    up to 16); lanes 64k + l take quartet (64k + l) / S, with its own bra and
     ket pair (both read from the linear view, per-lane gathers).  Lane
     sub = l % S sums primitive quartets sub, sub + S, ... (bra-major order,
     as the reference's loops); the S partial sums are added across the
@@ -469,7 +469,7 @@ def emit_lpq_list(am, w):
     w('extern "C" hipError_t simint_launch_lpql_%s(const EriListArgs *a, hipStream_t s)' % name)
     w("{")
     w("    if (a->nquartet <= 0) return hipSuccess;")
-    w("    if (a->split < 1 || a->split > 8 || (a->split & (a->split - 1))) return hipErrorInvalidValue;")
+    w("    if (a->split < 1 || a->split > 16 || (a->split & (a->split - 1))) return hipErrorInvalidValue;")
     w("    hipLaunchKernelGGL(eri_lpql_%s, dim3((unsigned)((a->nquartet * a->split + 63) / 64)), dim3(64), 0, s, *a);"
       % name)
     w("    return hipGetLastError();")

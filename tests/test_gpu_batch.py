@@ -100,9 +100,9 @@ def test_list_both_families(gpu, family):
     b.free()
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
 def test_list_lanes_per_quartet(gpu, lanes):
-    """Lane-per-quartet list kernels with 1, 2, 4 or 8 lanes per quartet
+    """Lane-per-quartet list kernels with 1, 2, 4, 8 or 16 lanes per quartet
     (each lane a strided share of the primitive quartets, shares added across
     the lanes) against the oracle on the all-classes stream."""
     import torch
