@@ -384,3 +384,115 @@ int md_eri_deriv1_terms(int nsh, const int *am, const int *nprim, const int *pri
 }
 
 double md_boys_value(int n, double x) { return (double)md_boys(n, (ld)x); }
+
+/* -------------------------------------------------------------------------
+ * One-electron integrals in long double, from the Hermite expansion
+ * (Helgaker, Jorgensen & Olsen, ch. 9): overlap S = E^{ij}_0 (pi/p)^{1/2} per
+ * direction; kinetic energy from the 1D identity
+ *   T_ij = -2 b^2 S_{i,j+2} + b (2j+1) S_ij - j(j-1)/2 S_{i,j-2};
+ * nuclear attraction -Z 2 pi/p e^{-mu AB^2} sum_tuv E_t E_u E_v R_tuv(p, P-C).
+ * Independent of the Obara-Saika tables the oracle and the kernels use.
+ * kind 0 overlap, 1 kinetic, 2 potential; blocks as oracle_osoei.
+ * ------------------------------------------------------------------------- */
+int md_oei_pairs(int kind, int nsh, const int *am, const int *nprim, const int *prim_off,
+                 const double *xyz, const double *alpha, const double *coef, int npair, const int *pairs,
+                 const long long *offsets, int ncenter, const double *Z, const double *cxyz, double *out)
+{
+    md_shell *s = (md_shell *)malloc(sizeof(md_shell) * (size_t)nsh);
+    load(nsh, am, nprim, prim_off, xyz, alpha, coef, s);
+    const ld pi = 3.14159265358979323846264338327950288L;
+    static __thread ld E[3][MD_MAXL + 2][MD_MAXL + 2][2 * MD_MAXL + 5];
+    for (int n = 0; n < npair; n++) {
+        const md_shell *A = &s[pairs[2 * n]], *B = &s[pairs[2 * n + 1]];
+        if (A->am + 2 > MD_MAXL || B->am + 2 > MD_MAXL) { free(s); return -1; }
+        const int na = ncart(A->am), nb = ncart(B->am);
+        int ea[64][3], eb[64][3];
+        cart_list(A->am, ea);
+        cart_list(B->am, eb);
+        ld acc[64 * 64];
+        for (int k = 0; k < na * nb; k++) acc[k] = 0.0L;
+        for (int i = 0; i < A->nprim; i++)
+            for (int j = 0; j < B->nprim; j++) {
+                const ld a = A->alpha[i], b = B->alpha[j], p = a + b, mu = a * b / p;
+                const ld cc = (ld)A->coef[i] * B->coef[j];
+                ld P[3], K[3];
+                for (int x = 0; x < 3; x++) {
+                    P[x] = (a * A->c[x] + b * B->c[x]) / p;
+                    const ld X = A->c[x] - B->c[x];
+                    K[x] = expl(-mu * X * X);
+                    hermite_E(A->am, B->am + 2, p, P[x] - A->c[x], P[x] - B->c[x], E[x]);
+                }
+                if (kind < 2) {
+                    const ld sq = sqrtl(pi / p);
+                    for (int u = 0; u < na; u++)
+                        for (int v = 0; v < nb; v++) {
+                            ld S1[3], T1[3];
+                            for (int x = 0; x < 3; x++) {
+                                const int ii = ea[u][x], jj = eb[v][x];
+                                S1[x] = E[x][ii][jj][0] * sq * K[x];
+                                if (kind == 1) {
+                                    ld t = -2.0L * b * b * E[x][ii][jj + 2][0] + b * (2 * jj + 1) * E[x][ii][jj][0];
+                                    if (jj >= 2) t -= 0.5L * jj * (jj - 1) * E[x][ii][jj - 2][0];
+                                    T1[x] = t * sq * K[x];
+                                }
+                            }
+                            ld val = kind == 0 ? S1[0] * S1[1] * S1[2]
+                                               : T1[0] * S1[1] * S1[2] + S1[0] * T1[1] * S1[2] + S1[0] * S1[1] * T1[2];
+                            acc[u * nb + v] += cc * val;
+                        }
+                    continue;
+                }
+                const int N = A->am + B->am, D = N + 1;
+                ld *Rn = (ld *)malloc(sizeof(ld) * D * D * D), *Rn1 = (ld *)malloc(sizeof(ld) * D * D * D);
+#define RI(t, u, v) (((t) * D + (u)) * D + (v))
+                for (int c = 0; c < ncenter; c++) {
+                    ld PC[3], R2 = 0.0L;
+                    for (int x = 0; x < 3; x++) { PC[x] = P[x] - (ld)cxyz[3 * c + x]; R2 += PC[x] * PC[x]; }
+                    const ld T = p * R2;
+                    double Fref[64];
+                    if (g_refboys) oracle_boys((double)T, N, Fref);
+                    for (int nn = N; nn >= 0; nn--) {
+                        const int lim = N - nn;
+                        ld m2p = 1.0L;
+                        for (int z = 0; z < nn; z++) m2p *= -2.0L * p;
+                        for (int t = 0; t <= lim; t++)
+                            for (int u = 0; u + t <= lim; u++)
+                                for (int v = 0; v + u + t <= lim; v++) {
+                                    ld r;
+                                    if (t > 0) {
+                                        r = PC[0] * Rn1[RI(t - 1, u, v)];
+                                        if (t > 1) r += (t - 1) * Rn1[RI(t - 2, u, v)];
+                                    } else if (u > 0) {
+                                        r = PC[1] * Rn1[RI(t, u - 1, v)];
+                                        if (u > 1) r += (u - 1) * Rn1[RI(t, u - 2, v)];
+                                    } else if (v > 0) {
+                                        r = PC[2] * Rn1[RI(t, u, v - 1)];
+                                        if (v > 1) r += (v - 1) * Rn1[RI(t, u, v - 2)];
+                                    } else {
+                                        r = m2p * (g_refboys ? (ld)Fref[nn] : md_boys(nn, T));
+                                    }
+                                    Rn[RI(t, u, v)] = r;
+                                }
+                        ld *tmp = Rn; Rn = Rn1; Rn1 = tmp;
+                    }
+                    const ld pre = -(ld)Z[c] * cc * 2.0L * pi / p * K[0] * K[1] * K[2];
+                    for (int u = 0; u < na; u++)
+                        for (int v = 0; v < nb; v++) {
+                            ld sum = 0.0L;
+                            for (int t = 0; t <= ea[u][0] + eb[v][0]; t++)
+                                for (int uu = 0; uu <= ea[u][1] + eb[v][1]; uu++)
+                                    for (int vv = 0; vv <= ea[u][2] + eb[v][2]; vv++)
+                                        sum += E[0][ea[u][0]][eb[v][0]][t] * E[1][ea[u][1]][eb[v][1]][uu] *
+                                               E[2][ea[u][2]][eb[v][2]][vv] * Rn1[RI(t, uu, vv)];
+                            acc[u * nb + v] += pre * sum;
+                        }
+                }
+#undef RI
+                free(Rn);
+                free(Rn1);
+            }
+        for (int k = 0; k < na * nb; k++) out[offsets[n] + k] = (double)acc[k];
+    }
+    free(s);
+    return 0;
+}

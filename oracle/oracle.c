@@ -1076,3 +1076,278 @@ int oracle_class_counts(int la, int lb, int lc, int ld, long long *counts)
     free(comps.t);
     return 0;
 }
+
+/* -------------------------------------------------------------------------
+ * fast-Schwarz primitive value exposed on its own, so it can be pinned
+ * against the exact (ij|ij) of the s-type primitive pair
+ * ------------------------------------------------------------------------- */
+double oracle_fastschwarz(int nsh, const int *am, const int *nprim, const int *prim_off,
+                          const double *xyz, const double *alpha, const double *coef,
+                          int sa, int sb, int i, int j)
+{
+    or_shell *s = (or_shell *)malloc(sizeof(or_shell) * (size_t)nsh);
+    load_shells(nsh, am, nprim, prim_off, xyz, alpha, coef, s);
+    const double v = fastschwarz_value(&s[sa], &s[sb], i, j);
+    free(s);
+    return v;
+}
+
+/* -------------------------------------------------------------------------
+ * One-electron integrals (skel/simint/osoei/osoei_{terms,overlap,ke,potential}.c)
+ *
+ * Cartesian order and recurrence directions are those of the reference's
+ * recur_lookup tables (python/gen_recurinfo.py: Psi4 order; a component is
+ * built by stepping the direction of its smallest non-zero exponent,
+ * rightmost on ties).
+ * ------------------------------------------------------------------------- */
+#define OEI_MAXL 7
+typedef struct { int e[3]; int dir; int dn1[3]; int dn2[3]; } oei_cart;
+static oei_cart g_oc[OEI_MAXL + 1][(OEI_MAXL + 1) * (OEI_MAXL + 2) / 2];
+
+static int oei_find(int l, int x, int y, int z)
+{
+    if (x < 0 || y < 0 || z < 0 || l < 0) return -1;
+    return (l - x) * (l - x + 1) / 2 + (l - x - y);
+}
+
+static void oei_tables(void)
+{
+    static int done = 0;
+    if (done) return;
+    for (int l = 0; l <= OEI_MAXL; l++) {
+        int n = 0;
+        for (int x = l; x >= 0; x--)
+            for (int y = l - x; y >= 0; y--, n++) {
+                oei_cart *c = &g_oc[l][n];
+                c->e[0] = x; c->e[1] = y; c->e[2] = l - x - y;
+                c->dir = -1;
+                if (l > 0) {
+                    int m = 99;
+                    for (int d = 0; d < 3; d++)
+                        if (c->e[d] > 0 && c->e[d] < m) m = c->e[d];
+                    for (int d = 2; d >= 0; d--)
+                        if (c->e[d] == m) { c->dir = d; break; }
+                }
+                for (int d = 0; d < 3; d++) {
+                    int e1[3] = {c->e[0], c->e[1], c->e[2]}, e2[3] = {c->e[0], c->e[1], c->e[2]};
+                    e1[d] -= 1; e2[d] -= 2;
+                    c->dn1[d] = oei_find(l - 1, e1[0], e1[1], e1[2]);
+                    c->dn2[d] = oei_find(l - 2, e2[0], e2[1], e2[2]);
+                }
+            }
+    }
+    done = 1;
+}
+
+/* osoei_terms.c:7-78 -- 1D overlap tables S_ij, i < nam1, j < nam2, per direction */
+static void oei_overlap_terms(double a1, const double *A, double a2, const double *B, int nam1, int nam2,
+                              double *terms)
+{
+    const double oop = 1.0 / (a1 + a2);
+    const double mu = a1 * a2 * oop;
+    const double oo2p = 0.5 * oop;
+    for (int d = 0; d < 3; d++) {
+        const double AB = A[d] - B[d];
+        const double P = (a1 * A[d] + a2 * B[d]) * oop;
+        const double PA = P - A[d], PB = P - B[d];
+        double *s = terms + d * nam1 * nam2;
+#define S_(i, j) s[(i) * nam2 + (j)]
+        S_(0, 0) = sqrt(3.14159265358979323846 * oop) * exp(-mu * AB * AB);
+        for (int i = 1; i < nam1; i++) {
+            S_(i, 0) = PA * S_(i - 1, 0);
+            if (i > 1) S_(i, 0) += (i - 1) * oo2p * S_(i - 2, 0);
+        }
+        for (int j = 1; j < nam2; j++) {
+            S_(0, j) = PB * S_(0, j - 1);
+            if (j > 1) S_(0, j) += (j - 1) * oo2p * S_(0, j - 2);
+        }
+        for (int i = 1; i < nam1; i++)
+            for (int j = 1; j < nam2; j++) {
+                S_(i, j) = PB * S_(i, j - 1) + oo2p * i * S_(i - 1, j - 1);
+                if (j > 1) S_(i, j) += oo2p * (j - 1) * S_(i, j - 2);
+            }
+#undef S_
+    }
+}
+
+/* osoei_terms.c:81-166 -- the overlap tables (first 3 blocks) and the 1D
+ * kinetic tables T_ij (blocks 3..5) */
+static void oei_ke_terms(double a1, const double *A, double a2, const double *B, int nam1, int nam2,
+                         double *terms)
+{
+    const double oop = 1.0 / (a1 + a2);
+    const double mu = a1 * a2 * oop;
+    const double oo2p = 0.5 * oop;
+    const double a1sq = 2.0 * a1 * a1;
+    for (int d = 0; d < 3; d++) {
+        const double AB = A[d] - B[d];
+        const double P = (a1 * A[d] + a2 * B[d]) * oop;
+        const double PA = P - A[d], PB = P - B[d];
+        double *s = terms + d * nam1 * nam2;
+        double *t = terms + (d + 3) * nam1 * nam2;
+#define S_(i, j) s[(i) * nam2 + (j)]
+#define T_(i, j) t[(i) * nam2 + (j)]
+        S_(0, 0) = sqrt(3.14159265358979323846 * oop) * exp(-mu * AB * AB);
+        T_(0, 0) = S_(0, 0) * (a1 - a1sq * (PA * PA + oo2p));
+        for (int i = 1; i < nam1; i++) {
+            S_(i, 0) = PA * S_(i - 1, 0);
+            if (i > 1) S_(i, 0) += (i - 1) * oo2p * S_(i - 2, 0);
+            T_(i, 0) = PA * T_(i - 1, 0) + 2 * mu * S_(i, 0);
+            if (i > 1) T_(i, 0) += (i - 1) * oo2p * T_(i - 2, 0) - a2 * oop * (i - 1) * S_(i - 2, 0);
+        }
+        for (int j = 1; j < nam2; j++) {
+            S_(0, j) = PB * S_(0, j - 1);
+            if (j > 1) S_(0, j) += (j - 1) * oo2p * S_(0, j - 2);
+            T_(0, j) = PB * T_(0, j - 1) + 2 * mu * S_(0, j);
+            if (j > 1) T_(0, j) += (j - 1) * oo2p * T_(0, j - 2) - a1 * oop * (j - 1) * S_(0, j - 2);
+        }
+        for (int i = 1; i < nam1; i++)
+            for (int j = 1; j < nam2; j++) {
+                S_(i, j) = PB * S_(i, j - 1) + oo2p * i * S_(i - 1, j - 1);
+                if (j > 1) S_(i, j) += oo2p * (j - 1) * S_(i, j - 2);
+                T_(i, j) = PB * T_(i, j - 1) + oo2p * i * T_(i - 1, j - 1) + 2 * mu * S_(i, j);
+                if (j > 1) T_(i, j) += oo2p * (j - 1) * T_(i, j - 2) - a1 * oop * (j - 1) * S_(i, j - 2);
+            }
+#undef S_
+#undef T_
+    }
+}
+
+/* osoei_overlap.c:10-66 (kind 0) and osoei_ke.c:10-67 (kind 1): one block
+ * of ncart(a) x ncart(b), primitive sums of products of the 1D tables */
+static void oei_sk(int kind, const or_shell *A, const or_shell *B, double *out)
+{
+    const int nam1 = A->am + 1, nam2 = B->am + 1, n12 = nam1 * nam2;
+    const int na = ncart(A->am), nb = ncart(B->am);
+    double w[6 * (OEI_MAXL + 2) * (OEI_MAXL + 2)];
+    const double xa[3] = {A->x, A->y, A->z}, xb[3] = {B->x, B->y, B->z};
+    for (int k = 0; k < na * nb; k++) out[k] = 0.0;
+    for (int i = 0; i < A->nprim; i++)
+        for (int j = 0; j < B->nprim; j++) {
+            const double pf = A->coef[i] * B->coef[j];
+            if (kind == 0) oei_overlap_terms(A->alpha[i], xa, B->alpha[j], xb, nam1, nam2, w);
+            else oei_ke_terms(A->alpha[i], xa, B->alpha[j], xb, nam1, nam2, w);
+            int o = 0;
+            for (int n = 0; n < na; n++)
+                for (int m = 0; m < nb; m++, o++) {
+                    const int *e1 = g_oc[A->am][n].e, *e2 = g_oc[B->am][m].e;
+                    const int ix = e1[0] * nam2 + e2[0], iy = e1[1] * nam2 + e2[1], iz = e1[2] * nam2 + e2[2];
+                    double v;
+                    if (kind == 0) v = w[ix] * w[n12 + iy] * w[2 * n12 + iz];
+                    else
+                        v = w[3 * n12 + ix] * w[n12 + iy] * w[2 * n12 + iz] +
+                            w[ix] * w[4 * n12 + iy] * w[2 * n12 + iz] +
+                            w[ix] * w[n12 + iy] * w[5 * n12 + iz];
+                    out[o] += pf * v;
+                }
+        }
+}
+
+/* osoei_potential.c:11-218 -- nuclear attraction, -sum_C Z_C <a|1/r_C|b>:
+ * per point charge and primitive pair, Theta^(m)_{i,0} up the first centre
+ * from 2 pi / p e^{-mu AB^2} F_m(p PC^2), then Theta^(m)_{i,j} up the second
+ * centre, each component along its recurrence direction. */
+static void oei_potential(int ncenter, const double *Z, const double *cxyz, const or_shell *A,
+                          const or_shell *B, double *out)
+{
+    const int am1 = A->am, am2 = B->am, L = am1 + am2;
+    const int na = ncart(am1), nb = ncart(am2);
+    /* th[i][j]: (M(i,j)+1) levels of ncart(i)*ncart(j) values, M(i,j) = L - i - j */
+    double *th[OEI_MAXL + 1][OEI_MAXL + 1];
+    size_t tot = 0;
+    for (int i = 0; i <= am1; i++)
+        for (int j = 0; j <= am2; j++) tot += (size_t)(L - i - j + 1) * ncart(i) * ncart(j);
+    double *thbuf = (double *)malloc(sizeof(double) * tot);
+    tot = 0;
+    for (int i = 0; i <= am1; i++)
+        for (int j = 0; j <= am2; j++) {
+            th[i][j] = thbuf + tot;
+            tot += (size_t)(L - i - j + 1) * ncart(i) * ncart(j);
+        }
+    const double xa[3] = {A->x, A->y, A->z}, xb[3] = {B->x, B->y, B->z};
+    double AB2 = 0.0;
+    for (int d = 0; d < 3; d++) AB2 += (xa[d] - xb[d]) * (xa[d] - xb[d]);
+    for (int k = 0; k < na * nb; k++) out[k] = 0.0;
+    for (int c = 0; c < ncenter; c++)
+        for (int ia = 0; ia < A->nprim; ia++)
+            for (int ib = 0; ib < B->nprim; ib++) {
+                const double a1 = A->alpha[ia], a2 = B->alpha[ib];
+                const double p = a1 + a2, oop = 1.0 / p, mu = a1 * a2 * oop, oo2p = 0.5 * oop;
+                double PA[3], PB[3], PC[3], PC2 = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    const double P = (a1 * xa[d] + a2 * xb[d]) * oop;
+                    PA[d] = P - xa[d];
+                    PB[d] = P - xb[d];
+                    PC[d] = P - cxyz[3 * c + d];
+                    PC2 += PC[d] * PC[d];
+                }
+                double F[2 * OEI_MAXL + 2];
+                oracle_boys(PC2 * p, L, F);
+                const double pre = 2.0 * 3.14159265358979323846 * oop * exp(-mu * AB2);
+                for (int m = 0; m <= L; m++) th[0][0][m] = F[m] * pre;
+                for (int i = 0; i <= am1; i++) {
+                    const int ni = ncart(i), ni1 = i > 0 ? ncart(i - 1) : 0, ni2 = i > 1 ? ncart(i - 2) : 0;
+                    if (i > 0) {
+                        for (int m = 0; m <= L - i; m++)
+                            for (int n = 0; n < ni; n++) {
+                                const oei_cart *cc = &g_oc[i][n];
+                                const int d = cc->dir;
+                                double v = PA[d] * th[i - 1][0][m * ni1 + cc->dn1[d]] -
+                                           PC[d] * th[i - 1][0][(m + 1) * ni1 + cc->dn1[d]];
+                                if (cc->e[d] > 1)
+                                    v += oo2p * (cc->e[d] - 1) *
+                                         (th[i - 2][0][m * ni2 + cc->dn2[d]] - th[i - 2][0][(m + 1) * ni2 + cc->dn2[d]]);
+                                th[i][0][m * ni + n] = v;
+                            }
+                    }
+                    for (int j = 1; j <= am2; j++) {
+                        const int nj = ncart(j), nj1 = ncart(j - 1), nj2 = j > 1 ? ncart(j - 2) : 0;
+                        for (int m = 0; m <= L - i - j; m++)
+                            for (int n = 0; n < ni; n++)
+                                for (int o = 0; o < nj; o++) {
+                                    const oei_cart *cb = &g_oc[j][o];
+                                    const int d = cb->dir;
+                                    const int ei = g_oc[i][n].e[d];
+                                    const int i1 = cb->dn1[d];
+                                    double v = PB[d] * th[i][j - 1][(m * ni + n) * nj1 + i1] -
+                                               PC[d] * th[i][j - 1][((m + 1) * ni + n) * nj1 + i1];
+                                    if (ei > 0) {
+                                        const int a1i = g_oc[i][n].dn1[d];
+                                        v += oo2p * ei * th[i - 1][j - 1][(m * ni1 + a1i) * nj1 + i1] -
+                                             oo2p * ei * th[i - 1][j - 1][((m + 1) * ni1 + a1i) * nj1 + i1];
+                                    }
+                                    if (cb->e[d] > 1) {
+                                        const int i2 = cb->dn2[d];
+                                        v += oo2p * (cb->e[d] - 1) *
+                                             (th[i][j - 2][(m * ni + n) * nj2 + i2] -
+                                              th[i][j - 2][((m + 1) * ni + n) * nj2 + i2]);
+                                    }
+                                    th[i][j][(m * ni + n) * nj + o] = v;
+                                }
+                    }
+                }
+                const double f = Z[c] * A->coef[ia] * B->coef[ib];
+                for (int k = 0; k < na * nb; k++) out[k] -= th[am1][am2][k] * f;
+            }
+    free(thbuf);
+}
+
+/* kind 0 overlap, 1 kinetic energy, 2 nuclear attraction of the point
+ * charges (Z, cxyz); block n = ncart(a) x ncart(b) values at out + offsets[n] */
+int oracle_osoei(int kind, int nsh, const int *am, const int *nprim, const int *prim_off,
+                 const double *xyz, const double *alpha, const double *coef, int npair, const int *pairs,
+                 const long long *offsets, int ncenter, const double *Z, const double *cxyz, double *out)
+{
+    ensure_init();
+    oei_tables();
+    or_shell *s = (or_shell *)malloc(sizeof(or_shell) * (size_t)nsh);
+    load_shells(nsh, am, nprim, prim_off, xyz, alpha, coef, s);
+    for (int n = 0; n < npair; n++) {
+        const or_shell *A = &s[pairs[2 * n]], *B = &s[pairs[2 * n + 1]];
+        if (A->am > OEI_MAXL || B->am > OEI_MAXL) { free(s); return -1; }
+        if (kind == 2) oei_potential(ncenter, Z, cxyz, A, B, out + offsets[n]);
+        else oei_sk(kind, A, B, out + offsets[n]);
+    }
+    free(s);
+    return 0;
+}

@@ -44,6 +44,12 @@ def lib():
         L.md_set_reference_boys.argtypes = [ctypes.c_int]
         L.md_boys_value.argtypes = [ctypes.c_int, ctypes.c_double]
         L.md_boys_value.restype = ctypes.c_double
+        L.oracle_fastschwarz.argtypes = [ctypes.c_int, ip, ip, ip, dp, dp, dp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]
+        L.oracle_fastschwarz.restype = ctypes.c_double
+        L.oracle_osoei.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip, dp, dp, dp, ctypes.c_int, ip, llp,
+                                   ctypes.c_int, dp, dp, dp]
+        L.md_oei_pairs.argtypes = L.oracle_osoei.argtypes
         _LIB = L
     return _LIB
 
@@ -179,3 +185,45 @@ def class_counts(la, lb, lc, ld):
     c = (ctypes.c_longlong * 4)()
     lib().oracle_class_counts(la, lb, lc, ld, c)
     return list(c)
+
+
+def fastschwarz(sa, ia, ib, i, j):
+    """The fast-Schwarz value of primitive pair (i of shell ia, j of shell ib)
+    (shell_screen.c:226-270): the s-type (ij|ij), not square-rooted."""
+    return lib().oracle_fastschwarz(*sa.args(), ia, ib, i, j)
+
+
+def _oei(fn, kind, sa, pairs, Z=None, cxyz=None):
+    pr = np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+    am = sa.am[pr]
+    n = ((am + 1) * (am + 2) // 2).prod(axis=1)
+    off = np.zeros(len(pr), dtype=np.int64)
+    if len(pr) > 1:
+        off[1:] = np.cumsum(n)[:-1]
+    out = np.zeros(max(int(n.sum()), 1), dtype=np.float64)
+    Zc = np.ascontiguousarray(np.zeros(1) if Z is None else Z, dtype=np.float64)
+    C = np.ascontiguousarray(np.zeros((1, 3)) if cxyz is None else cxyz, dtype=np.float64).reshape(-1, 3)
+    nc = 0 if Z is None else len(Zc)
+    rc = fn(kind, *sa.args(), len(pr), _p(pr, ctypes.c_int), _p(off, ctypes.c_longlong), nc,
+            _p(Zc, ctypes.c_double), _p(C, ctypes.c_double), _p(out, ctypes.c_double))
+    if rc != 0:
+        raise ValueError("one-electron integrals: angular momentum too high")
+    return out[:int(n.sum())], off, n
+
+
+def osoei(sa, pairs, kind, Z=None, cxyz=None):
+    """One-electron integrals restated from skel/simint/osoei: kind 0
+    overlap, 1 kinetic energy, 2 nuclear attraction of point charges Z at
+    cxyz; ncart(a) x ncart(b) per shell pair, concatenated."""
+    return _oei(lib().oracle_osoei, kind, sa, pairs, Z, cxyz)
+
+
+def md_oei(sa, pairs, kind, Z=None, cxyz=None, reference_boys=False):
+    """The same integrals in long double from the Hermite expansion (exact
+    Boys values, or simint's split scheme with reference_boys=True),
+    independent of the Obara-Saika restatement."""
+    lib().md_set_reference_boys(1 if reference_boys else 0)
+    try:
+        return _oei(lib().md_oei_pairs, kind, sa, pairs, Z, cxyz)
+    finally:
+        lib().md_set_reference_boys(0)

@@ -122,7 +122,8 @@ class QuartetBatch:
         import torch
         L = simint.lib()
         self._prepare(n)
-        main = torch.cuda.ExternalStream(L.simint_hip_stream())
+        prev = L.simint_hip_stream()     # the stream the caller's calls queue on (restored below)
+        main = torch.cuda.ExternalStream(prev)
         fork = torch.cuda.Event()
         fork.record(main)
         for s in self._side:
@@ -140,7 +141,7 @@ class QuartetBatch:
                     simint.compute_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(),
                                         d_out, self.screen_tol)
         finally:
-            L.simint_hip_set_stream(None)
+            L.simint_hip_set_stream(ctypes.c_void_p(prev))
         for s in self._side:
             e = torch.cuda.Event()
             e.record(s)

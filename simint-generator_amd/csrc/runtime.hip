@@ -417,6 +417,15 @@ int validate(const simint_multi_shellpair *P)
     return 0;
 }
 
+// A first-derivative class needs its (a+1) classes and an assembly table:
+// both exist up to SIMINT_OSTEI_DERIV1_MAXAM on every centre.  Checked before
+// anything is queued, so an unsupported class leaves the stream untouched.
+bool deriv1_built(int a, int b, int c, int d)
+{
+    return std::max(std::max(a, b), std::max(c, d)) <= SIMINT_OSTEI_DERIV1_MAXAM &&
+           std::min(std::min(a, b), std::min(c, d)) >= 0;
+}
+
 // ----------------------------------------------------- first derivatives
 // d/dX_i (ab|cd) for the three explicit centres from the 2a-weighted (a+1)
 // class and the (a-1) class, and the fourth centre (the first with the
@@ -557,6 +566,10 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
         t_err = "derivative order not built";
         return SIMINT_ERROR_UNSUPPORTED;
     }
+    if (deriv == 1 && !deriv1_built(P->am1, P->am2, Q->am1, Q->am2)) {
+        t_err = "first derivatives not built for this class";
+        return SIMINT_ERROR_UNSUPPORTED;
+    }
     // shell-level screen (ostei.c:17-20): screen values are not square-rooted
     const double tol2 = screen_tol * screen_tol;
     if (screen_tol > 0.0 && (P->screen_max * Q->screen_max) < tol2) return -1;
@@ -634,6 +647,9 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
 
     ThreadCtx &c = ctx();
     hipError_t e;
+    // an earlier asynchronous call of this thread (_range, _prefix, _diag)
+    // may still read c.stage, c.in or c.split: wait for it before reuse
+    if ((e = hipStreamSynchronize(c.s)) != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
     const size_t final_n = (size_t)nP * nQ * nc * (deriv ? 12 : 1);
     const size_t out_bytes = sizeof(double) * (scratch + (deriv ? final_n : 0));
     if ((e = c.stage.reserve(im.buf.size())) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
@@ -668,7 +684,11 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
         a.tol2 = (screen_tol > 0.0 && P->screen && Q->screen) ? tol2 : 0.0;
         a.ket_screen_max = Q->screen_max;
         a.stride = t.nc;
-        if ((e = t.fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
+        if ((e = t.fn(&a, c.s)) != hipSuccess) {
+            set_err("class kernel launch", e);
+            hipStreamSynchronize(c.s);   // the call returns with nothing of it still queued
+            return SIMINT_ERROR_HIP;
+        }
     }
     const double *result = dout;
     if (deriv == 1) {
@@ -686,11 +706,19 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
         d.nquartet = (long long)nP * nQ;
         d.out = dout + scratch;
         d.qout = nullptr;
-        if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
+        if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) {
+            set_err("deriv1 assembly", e);
+            hipStreamSynchronize(c.s);
+            return SIMINT_ERROR_HIP;
+        }
         result = d.out;
     }
     e = hipMemcpyAsync(integrals, result, sizeof(double) * final_n, hipMemcpyDeviceToHost, c.s);
-    if (e != hipSuccess) { set_err("hipMemcpyAsync(D2H)", e); return SIMINT_ERROR_HIP; }
+    if (e != hipSuccess) {
+        set_err("hipMemcpyAsync(D2H)", e);
+        hipStreamSynchronize(c.s);
+        return SIMINT_ERROR_HIP;
+    }
     e = hipStreamSynchronize(c.s);
     if (e != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
     return nP * nQ;
@@ -1074,6 +1102,10 @@ long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const si
     if (!P || !Q || nquartet < 0 || (nquartet > 0 && (!d_qbra || !d_qket || !d_out || !d_work)))
         return SIMINT_ERROR_ARGUMENT;
     if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    if (!deriv1_built(P->am1, P->am2, Q->am1, Q->am2)) {
+        t_err = "first derivatives not built for this class";
+        return SIMINT_ERROR_UNSUPPORTED;
+    }
     const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
     const DerivPlan dp = deriv1_plan(am);
     simint_list_launch_fn fns[8];
@@ -1123,6 +1155,10 @@ long long simint_hip_compute_eri_deriv1_range(const simint_hip_pairs *P, const s
 {
     if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n || !d_work || !d_out) return SIMINT_ERROR_ARGUMENT;
     if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    if (!deriv1_built(P->am1, P->am2, Q->am1, Q->am2)) {
+        t_err = "first derivatives not built for this class";
+        return SIMINT_ERROR_UNSUPPORTED;
+    }
     const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
     const DerivPlan dp = deriv1_plan(am);
     RangeTerm terms[8];

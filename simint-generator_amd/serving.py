@@ -149,9 +149,11 @@ class Server:
         work = None
         if deriv:
             work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
-        s = torch.zeros(2, dtype=torch.float64, device="cuda")
         t_note = time.perf_counter()
         with torch.cuda.stream(stream):
+            # zeroed on the library stream, so the fill is ordered before the
+            # accumulations that stream runs
+            s = torch.zeros(2, dtype=torch.float64, device="cuda")
             for c in range(ss.nchunk):
                 if self.verbose and time.perf_counter() - t_note > 30.0:
                     print("[serve %.1fs] request %d: chunk %d of %d" % (self.now(), r.rid, c, ss.nchunk),

@@ -57,6 +57,7 @@ def test_list_mixed_stream_all_classes(gpu):
     b = batch.QuartetBatch(basis, qs)
     assert len(b.groups) == 256
     out = torch.full((b.out_doubles,), float("nan"), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()   # the fill (torch's stream) ends before the library stream writes
     b.run(out.data_ptr())
     simint.synchronize()
     got = out.cpu().numpy()

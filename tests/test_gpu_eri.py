@@ -335,6 +335,7 @@ def test_split_kernels_at_scale(gpu):
             if L.simint_hip_set_class_kernel(*am, fam) < 0:
                 continue
             out = torch.full((len(pairs) * nab * nab,), float("nan"), dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()   # fill done before the library stream writes
             simint.compute_diag(dev, out.data_ptr())
             simint.synchronize()
             res[fam] = out.cpu().numpy()

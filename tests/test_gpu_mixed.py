@@ -22,7 +22,9 @@ ABS_BAR = 1e-10
 def _chunk_outputs(ss, chunk, out):
     """Run one output chunk of a screened set; returns {launch index: (lo,
     hi) of the chunk region it wrote} and the buffer on the host."""
+    import torch
     out.fill_(float("nan"))
+    torch.cuda.synchronize()   # fill done before the library stream writes
     ss.run_pass(out.data_ptr(), chunk=chunk)
     simint.synchronize()
     return out.cpu().numpy()
@@ -73,6 +75,7 @@ def test_mixed_vs_fp64_and_oracle(gpu):
         nc = simint.ncart(shells[q[0]][0]) * simint.ncart(shells[q[1]][0]) * simint.ncart(shells[q[2]][0]) \
             * simint.ncart(shells[q[3]][0])
         o = torch.full((nc,), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()   # fill done before the library stream writes
         simint.compute_range(Pd, Qd, 0, None, [1], [0], o.data_ptr(), 0.0, 1)
         simint.synchronize()
         got.append(o.cpu().numpy())
@@ -108,6 +111,7 @@ def test_range_matches_prefix(gpu):
         off = np.zeros(P.n, dtype=np.int64)
         off[1:] = np.cumsum(cnt[:-1]) * nc
         out = torch.full((int(cnt.sum()) * nc + 1,), float("nan"), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()   # fill done before the library stream writes
         for prec in (0, 1):
             simint.compute_range(P, Q, 0, lo, hi, off, out.data_ptr(), 0.0, prec)
             simint.synchronize()

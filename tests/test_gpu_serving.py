@@ -63,6 +63,7 @@ def test_screened_deriv_pass_vs_oracle(gpu):
     got, qs = [], []
     for c in range(ss.nchunk):
         out.fill_(float("nan"))
+        torch.cuda.synchronize()
         ss.run_pass(out.data_ptr(), chunk=c, work_ptr=work.data_ptr())
         simint.synchronize()
         host = out.cpu().numpy()

@@ -12,7 +12,12 @@ What fixes each part:
     the Boys-scheme error allowance;
   * invariants: bra/ket and centre permutations, translation, Schwarz bound;
   * first derivatives: Richardson-extrapolated finite differences of the
-    long-double brute force.
+    long-double brute force;
+  * normalisation: a normalised shell's x^l component has unit self-overlap
+    (long-double Hermite overlap);
+  * fast-Schwarz values: the exact (ij|ij) of the s-type primitive pair;
+  * one-electron integrals (overlap, kinetic energy, nuclear attraction):
+    long-double Hermite (McMurchie-Davidson) expansions, closed forms for s.
 """
 import ctypes
 import itertools
@@ -245,3 +250,101 @@ def test_md_analytic_deriv1_against_finite_differences():
     # the oracle (deriv = 1) is the same quantity
     ref, _, _ = oracle.eri(sa, qs, deriv=1)
     assert np.abs(ref - ex).max() <= 1e-13 * max(1.0, np.abs(ex).max())
+
+
+def _random_shells(seed, lmax=3, per_l=3, depth=(1, 6), lo=0.05, hi=50.0, box=2.0):
+    rng = np.random.default_rng(seed)
+    shells = []
+    for l in range(lmax + 1):
+        for _ in range(per_l):
+            n = int(rng.integers(depth[0], depth[1] + 1))
+            shells.append((l, tuple(rng.uniform(-box, box, 3)), list(np.exp(rng.uniform(np.log(lo), np.log(hi), n))),
+                           list(rng.uniform(0.1, 1.0, n))))
+    return shells
+
+
+def test_normalize_unit_self_overlap():
+    """oracle_normalize (shell.c:248-278): after normalisation the x^l
+    Cartesian of every shell has <x^l|x^l> = 1, computed independently in
+    long double.  A wrong exponent m, a missing norm_fac[l] or coefficients
+    left unscaled by alpha^(m/2) all move this by far more than 1e-13."""
+    shells = _random_shells(3, lmax=5, per_l=4)
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    d, off, n = oracle.md_oei(sa, [(i, i) for i in range(len(shells))], 0)
+    selfov = d[off]
+    assert np.all(np.abs(selfov - 1.0) < 1e-13), selfov
+    # and the unnormalised shells are not (the check has teeth)
+    raw = oracle.ShellArrays(shells)
+    d0, off0, _ = oracle.md_oei(raw, [(i, i) for i in range(len(shells))], 0)
+    assert np.abs(d0[off0] - 1.0).max() > 1e-2
+
+
+def test_fastschwarz_is_exact_s_type_diagonal():
+    """fastschwarz_value (shell_screen.c:226-270) is, for any shell AM, the
+    (ij|ij) integral of the s-type primitives i, j with their contraction
+    coefficients -- compared with the long-double brute force of that
+    single-primitive s quartet."""
+    shells = _random_shells(5, lmax=3, per_l=2, depth=(1, 4))
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    worst = 0.0
+    for ia in range(len(shells)):
+        for ib in range(len(shells)):
+            for i in range(sa.nprim[ia]):
+                for j in range(sa.nprim[ib]):
+                    v = oracle.fastschwarz(sa, ia, ib, i, j)
+                    s2 = [(0, tuple(sa.xyz[ia]), [sa.alpha[sa.off[ia] + i]], [sa.coef[sa.off[ia] + i]]),
+                          (0, tuple(sa.xyz[ib]), [sa.alpha[sa.off[ib] + j]], [sa.coef[sa.off[ib] + j]])]
+                    ex, _, _ = oracle.md_eri(oracle.ShellArrays(s2), [(0, 1, 0, 1)])
+                    # FP64 exp(-2 rho R^2) carries the rounding of its argument:
+                    # relative error ~ eps * (1 + 2 rho R^2)
+                    a, b = s2[0][2][0], s2[1][2][0]
+                    x = 2.0 * a * b / (a + b) * float(((sa.xyz[ia] - sa.xyz[ib]) ** 2).sum())
+                    worst = max(worst, abs(v - ex[0]) / abs(ex[0]) / (1.0 + x))
+    assert worst < 1e-14, worst
+
+
+def test_osoei_against_hermite():
+    """One-electron restatement (osoei_overlap/ke/potential.c) against the
+    long-double Hermite expansions: overlap and kinetic energy to 1e-14
+    normwise per block; nuclear attraction to 1e-14 with simint's Boys values
+    (algebra) and within the Boys scheme's own error with exact ones."""
+    shells = _random_shells(7, lmax=3, per_l=2, depth=(1, 4))
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    rng = np.random.default_rng(1)
+    Z = np.array([1.0, 6.0, 8.0, 92.0])
+    C = rng.uniform(-2.5, 2.5, (4, 3))
+    pairs = [(i, j) for i in range(len(shells)) for j in range(len(shells))]
+
+    def worst(a, b, off, n):
+        return max(np.abs(a[o:o + m] - b[o:o + m]).max() / np.abs(b[o:o + m]).max() for o, m in zip(off, n))
+    for kind in (0, 1):
+        a, off, n = oracle.osoei(sa, pairs, kind)
+        b, _, _ = oracle.md_oei(sa, pairs, kind)
+        assert worst(a, b, off, n) < 1e-14, (kind, worst(a, b, off, n))
+    a, off, n = oracle.osoei(sa, pairs, 2, Z, C)
+    b, _, _ = oracle.md_oei(sa, pairs, 2, Z, C, reference_boys=True)
+    assert worst(a, b, off, n) < 1e-14
+    b, _, _ = oracle.md_oei(sa, pairs, 2, Z, C)
+    assert worst(a, b, off, n) < 1e-11
+
+
+def test_osoei_s_closed_forms():
+    """s-type closed forms: S = (pi/p)^(3/2) e^{-mu R^2}, T = mu (3 - 2 mu
+    R^2) S, V = -Z 2 pi/p e^{-mu R^2} F_0(p |P-C|^2) (unit coefficients)."""
+    from math import erf, exp, pi, sqrt
+    a, b = 0.7, 1.9
+    A, B, Cc = np.array([0.1, -0.4, 0.3]), np.array([-0.5, 0.6, 1.1]), np.array([0.9, 0.2, -0.7])
+    sa = oracle.ShellArrays([(0, tuple(A), [a], [1.0]), (0, tuple(B), [b], [1.0])])
+    p, mu = a + b, a * b / (a + b)
+    R2 = float(((A - B) ** 2).sum())
+    S = (pi / p) ** 1.5 * exp(-mu * R2)
+    T = mu * (3.0 - 2.0 * mu * R2) * S
+    P = (a * A + b * B) / p
+    x = p * float(((P - Cc) ** 2).sum())
+    V = -3.0 * 2.0 * pi / p * exp(-mu * R2) * 0.5 * sqrt(pi / x) * erf(sqrt(x))
+    got = [oracle.osoei(sa, [(0, 1)], k, np.array([3.0]), Cc.reshape(1, 3))[0][0] for k in range(3)]
+    for g, e in zip(got, (S, T, V)):
+        assert abs(g - e) <= 1e-14 * abs(e), (g, e)
