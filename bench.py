@@ -1,29 +1,50 @@
 """Throughput benchmark of the MI355X ERI engine (one JSON line on stdout).
 
-Workload: the 64-atom, all-classes-up-to-d system (jittered 3-bohr cubic
-grid; per atom s(6) s(3) p(3) core and s(1) p(1) d(1) valence shells), every
-unique shell quartet passing a Schwarz screen of 1e-12, FP64.  One step is
-one pass over all of those quartets through the device-resident batched path
-(simint_hip_compute_eri_prefix), integrals written to HBM (a reused output
-buffer: the full set is ~0.85 TB of integrals).  Inputs are resident before
-the timed region.
+Main workload (`value`): the 256-atom system (jittered 3-bohr 8x8x4 grid; per
+atom s(6) s(3) p(3) core and s(1) p(1) d(1) valence shells; 1536 shells),
+every unique shell quartet passing a Schwarz screen of 1e-12 (1.2e11
+quartets), through the device-resident path in mixed precision: quartets
+whose Schwarz bound is below 1e-4 (divided by 400 with one HRR side and by
+1e5 with two) run the VRR in FP32, with the Boys function, prefactors,
+contraction and HRR in FP64; the others run FP64.  The same set through the
+plain FP64 path is timed beside it, and the maximum absolute error of the
+mixed path is measured per class against the FP64 GPU path (every quartet
+of a strided sample of output chunks) and against the CPU oracle (a located
+sample of FP32 quartets).
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--natom 64] [--no-cpu]
+A step is one share of that quartet set: the launch plan is cut into --steps
+consecutive shares of about equal quartet count, so --steps steps are one
+full pass whatever --steps is.  Integrals go to a reused 4 GiB output buffer
+in HBM (the whole set is ~70 TB of integrals); inputs are resident before
+the timed region, which is bracketed by device synchronisations.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-batched]
+                    [--no-serving] [--no-deriv] [--no-fp64-64]
 
 Besides the metric the line carries:
-  roofline      for the kernel with the largest share of device time:
-                algorithmic FP64 FLOPs per launch (FMA = 2; per quartet
-                K (f_setup + f_vrr + f_boys + bcont) + f_hrr, the reference
-                generator's operation counts) over its mean launch time,
-                measured with HIP events on the library stream in the timed
-                region, against the 78.6 TFLOP/s FP64 peak;
-  cpu_baseline  the CPU oracle (a C restatement of the reference algorithm,
-                one core) on a random sample of the same quartets.
+  fp64 / mixed      quartets/s of both paths, their ratio, per-class error;
+  roofline          the kernel with the largest share of device time in the
+                    timed mixed pass: algorithmic FP64 FLOPs (FMA = 2; per
+                    quartet K (f_setup + f_vrr + f_boys + bcont) + f_hrr, the
+                    reference generator's operation counts) or output bytes,
+                    whichever bounds it, over its mean launch time (HIP events
+                    on the library stream in the timed region);
+  whole_job         sum over launches of max(F64/P64 + F32/P32, bytes/BW)
+                    over the measured time (the job's roofline fraction);
+  serving           a bounded Poisson trace through the continuous-batching
+                    server: p50/p99 latency and quartets/s;
+  batched           1e6 quartets of all 256 classes up to f in one graph;
+  deriv1            first derivatives of the 64-atom system;
+  fp64_64atom       the 64-atom system in FP64 (the all-classes-to-d config);
+  cpu_baseline      the CPU oracle (a C restatement of the reference
+                    algorithm) on every host core the box gives this job,
+                    on a sample of the same quartets.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -31,7 +52,19 @@ PKG = os.path.join(ROOT, "simint-generator_amd")
 
 METRIC = "shell quartets/s at ≤1e-10 abs err, mixed vs FP64, 1 GPU; p50/p99 latency"
 PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
+T_START = time.perf_counter()
+BATCH_STREAMS = 16   # side streams the batched stream's classes are spread over
+
+
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print("[bench %.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
+
+
+def cname(cls):
+    return "_".join("spdfghik"[x] for x in cls)
 
 
 def measured_traffic(kernel):
@@ -51,62 +84,316 @@ def measured_traffic(kernel):
     return None
 
 
-def progress(msg):
-    """A progress line on stderr (the JSON result is the only stdout line)."""
-    print("[bench %.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
-
-
-T_START = time.perf_counter()
-BATCH_STREAMS = 16   # side streams the batched stream's classes are spread over (4 and 8 measured equal, 16 0-2 % faster)
-
-
-def kernel_symbol(info, cls):
-    """Name of the kernel the class runs now (the family selected at run
-    time; the split kernels carry their group count)."""
+def kernel_symbol(info, cls, precision=0):
+    """Name of the kernel a range launch of the class runs now (the family
+    selected at run time; the split kernels carry their group count)."""
     import simint
-    name = "_".join("spdfghik"[x] for x in cls)
+    name = cname(cls)
     fam = simint.lib().simint_hip_class_kernel(*cls)
     if fam == 1:
-        return "eri_lpq_%s" % name
+        return ("eri_lpqm_%s" if precision == 1 else "eri_lpq_%s") % name
     if fam in (3, 4):
         return "eri_" + info[name]["split_kernels"][fam - 3]
     return "eri_wpq_%s" % name
 
 
 def list_kernel_symbol(cls):
-    """Name of the quartet-list kernel the class runs now."""
     import simint
-    name = "_".join("spdfghik"[x] for x in cls)
-    return ("eri_lpql_%s" if simint.lib().simint_hip_class_list_kernel(*cls) == 1 else "eri_wpql_%s") % name
+    return ("eri_lpql_%s" if simint.lib().simint_hip_class_list_kernel(*cls) == 1 else "eri_wpql_%s") % cname(cls)
+
+
+def shares(launches, k):
+    """The plan cut into k consecutive shares of about equal quartet count."""
+    tot = max(sum(ln.nq for ln in launches), 1)
+    out = [[] for _ in range(k)]
+    acc = 0
+    for ln in launches:
+        out[min(k - 1, acc * k // tot)].append(ln)
+        acc += ln.nq
+    return out
+
+
+def launch_split_flops(info, ln):
+    """(FP64 flops, FP32 flops) of a range launch: the mixed kernels run
+    the VRR in FP32, everything else FP64."""
+    c = info[cname(ln.cls)]
+    K = ln.bra.nprim * ln.ket.nprim
+    if ln.precision == 1:
+        f32 = float(ln.nq) * K * c["f_vrr"]
+        return ln.flops - f32, f32
+    return ln.flops, 0.0
+
+
+class Timer:
+    """Per-launch HIP events on the library stream."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.ev = []
+
+    def __call__(self, ln, start):
+        import torch
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(self.stream)
+        self.ev.append((ln, e))
+
+    def per_launch(self):
+        return [(self.ev[k][0], self.ev[k][1].elapsed_time(self.ev[k + 1][1]) / 1000.0)
+                for k in range(0, len(self.ev), 2)]
+
+
+def timed_pass(ss, plan, steps, warmup, out, stream):
+    """Warm up on the first `warmup` shares, then time all `steps` shares of
+    the plan; returns (wall s, device s, per-launch seconds)."""
+    import torch
+    import simint
+    ss.launches = plan
+    sh = shares(plan, steps)
+    for s in range(min(warmup, steps)):
+        ss.run_launches(out.data_ptr(), sh[s])
+    simint.synchronize()
+    torch.cuda.synchronize()
+    tm = Timer(stream)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for s in range(steps):
+        ss.run_launches(out.data_ptr(), sh[s], events=tm)
+    e1.record(stream)
+    simint.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, e0.elapsed_time(e1) / 1000.0, tm.per_launch()
+
+
+def error_check(ss, plan64, planm, out, ref, max_chunks=96, per_class=40, seed=0):
+    """Max absolute error of the mixed path per class: (a) against the FP64
+    GPU path over every quartet of the sampled output chunks (the first two
+    chunks of each class holding FP32 quartets plus a stride over all such
+    chunks, at most max_chunks), (b) against the CPU oracle on `per_class`
+    located FP32 quartets of each class drawn from those chunks."""
+    import numpy as np
+    import oracle
+    import simint
+    ss.launches = planm
+    fp32_chunks = {}
+    for ln in planm:
+        if ln.precision == 1:
+            fp32_chunks.setdefault(ln.chunk, set()).add(ln.cls)
+    chunks = sorted(fp32_chunks)
+    pick = set()
+    seen = {}
+    for c in chunks:
+        for cls in fp32_chunks[c]:
+            if seen.get(cls, 0) < 2:
+                pick.add(c)
+                seen[cls] = seen.get(cls, 0) + 1
+    stride = max(1, len(chunks) // max(1, max_chunks - len(pick)))
+    pick |= set(chunks[::stride])
+    pick = sorted(pick)
+    dev_err = {}
+    probes = {}
+    mixed_launches = [ln for c in pick for ln in ss.chunk_launches(c) if ln.precision == 1]
+    bycls = {}
+    for ln in mixed_launches:
+        bycls.setdefault(ln.cls, []).append(ln)
+    for k, (cls, lns) in enumerate(sorted(bycls.items())):
+        for c, off, nc, q in ss.located_sample(per_class, seed=seed + k, launches=lns):
+            probes.setdefault(c, []).append((off, nc, q, cls))
+    got = []
+    t_note = time.perf_counter()
+    for i, c in enumerate(pick):
+        if time.perf_counter() - t_note > 20.0:
+            progress("error check: chunk %d of %d" % (i, len(pick)))
+            t_note = time.perf_counter()
+        n = ss.chunk_doubles(c)
+        ss.launches = plan64
+        ss.run_pass(ref.data_ptr(), chunk=c)
+        ss.launches = planm
+        ss.run_pass(out.data_ptr(), chunk=c)
+        simint.synchronize()
+        for ln in ss.chunk_launches(c):
+            if ln.precision != 1:
+                continue
+            # the FP32 launch's own region of the chunk
+            lo = np.zeros_like(ln.nket) if ln.lo is None else ln.lo
+            nc = ln.nout // ln.nq
+            a = int(ln.off[0])
+            b = int(ln.off[-1]) + (int(ln.nket[-1]) - int(lo[-1])) * nc
+            e = float((out[a:b] - ref[a:b]).abs().max()) if b > a else 0.0
+            dev_err[cname(ln.cls)] = max(dev_err.get(cname(ln.cls), 0.0), e)
+        for off, nc, q, cls in probes.get(c, []):
+            got.append((cls, q, out[off:off + nc].cpu().numpy()))
+    shells = ss.shells
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    orc_err = {}
+    for cls in sorted({g[0] for g in got}):
+        qs = [g[1] for g in got if g[0] == cls]
+        vals = np.concatenate([g[2] for g in got if g[0] == cls])
+        r, _, _ = oracle.eri(sa, qs)
+        orc_err[cname(cls)] = float(np.abs(vals - r).max())
+    ss.launches = planm
+    cover = sum(ln.nq for c in pick for ln in ss.chunk_launches(c) if ln.precision == 1)
+    return dev_err, orc_err, {"chunks_compared": len(pick), "fp32_quartets_compared": int(cover),
+                              "oracle_quartets": len(got)}
+
+
+def mixed_system_leg(args):
+    """The metric: the 256-atom system, mixed precision vs plain FP64."""
+    import torch
+    import quartets
+    import simint
+    import workloads
+    shells = workloads.config2(natom=args.natom, seed=0)
+    t0 = time.perf_counter()
+    ss = quartets.ScreenedSet(shells, tol=1e-12)
+    plan64 = ss.launches
+    ss.mixed = args.mixed_threshold
+    ss._plan()
+    planm = ss.launches
+    if args.only_class:
+        plan64 = [ln for ln in plan64 if cname(ln.cls) == args.only_class]
+        planm = [ln for ln in planm if cname(ln.cls) == args.only_class]
+    setup = time.perf_counter() - t0
+    nq = sum(ln.nq for ln in planm)
+    progress("%d-atom set: %d quartets, %d mixed launches, %d FP64 launches (setup %.1f s)"
+             % (args.natom, nq, len(planm), len(plan64), setup))
+    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
+    wm, dm, perm = timed_pass(ss, planm, args.steps, args.warmup, out, stream)
+    progress("mixed: %d steps in %.1f s" % (args.steps, wm))
+    w64, d64, per64 = timed_pass(ss, plan64, args.steps, args.warmup, out, stream)
+    progress("FP64: %d steps in %.1f s" % (args.steps, w64))
+    res = {"wall_mixed": wm, "wall_fp64": w64, "dev_mixed": dm, "dev_fp64": d64, "quartets": nq,
+           "setup_s": setup, "per_launch": perm, "per_launch64": per64,
+           "fp32_quartets": sum(ln.nq for ln in planm if ln.precision == 1)}
+    if not args.no_errcheck:
+        ref = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+        dev_err, orc_err, cov = error_check(ss, plan64, planm, out, ref)
+        progress("error check done")
+        res.update({"dev_err": dev_err, "orc_err": orc_err, "err_cover": cov})
+        del ref
+    res["info"] = ss.info
+    res["ss"] = ss
+    return res
+
+
+def roofline_of(per, info):
+    """Roofline object for the kernel (class, precision) with the largest
+    device time among per-launch timings."""
+    agg = {}
+    for ln, s in per:
+        key = (ln.cls, ln.precision)
+        a = agg.setdefault(key, {"s": 0.0, "f64": 0.0, "f32": 0.0, "bytes": 0.0, "n": 0})
+        f64, f32 = launch_split_flops(info, ln)
+        a["s"] += s
+        a["f64"] += f64
+        a["f32"] += f32
+        a["bytes"] += 8.0 * ln.nout
+        a["n"] += 1
+    tot = sum(a["s"] for a in agg.values())
+    key = max(agg, key=lambda k: agg[k]["s"])
+    a = agg[key]
+    mean_s = a["s"] / a["n"]
+    f64, f32, by = a["f64"] / a["n"], a["f32"] / a["n"], a["bytes"] / a["n"]
+    t_flop = f64 / PEAK_FP64 + f32 / PEAK_FP32
+    t_byte = by / PEAK_HBM
+    kname = kernel_symbol(info, key[0], key[1])
+    if t_flop >= t_byte:
+        # FP64 vector (VALU) bound: no MFMA is involved; "mfma" is the
+        # compute-bound label of the line format
+        ach = (f64 + f32) / mean_s / 1e12
+        pk = (f64 + f32) / (t_flop * 1e12)
+        roof = {"bound": "mfma", "achieved": ach, "peak": pk, "unit": "TFLOP/s", "frac": ach / pk}
+    else:
+        ach = by / mean_s / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / (PEAK_HBM / 1e9)}
+    roof.update({"traffic": measured_traffic(kname), "kernel": kname, "class": cname(key[0]),
+                 "precision": "mixed" if key[1] else "fp64",
+                 "compute": "vector ALU (no MFMA); compute peak is the FP64 vector rate (78.6 TFLOP/s) with "
+                            "FP32 flops priced at 157.3 TFLOP/s",
+                 "algorithmic_bytes_per_launch": by, "flops_per_launch": f64 + f32, "fp32_flops_per_launch": f32,
+                 "launches": a["n"], "mean_launch_ms": 1000.0 * mean_s, "share_of_device_time": a["s"] / tot})
+    return roof, agg
+
+
+def whole_job(per, info, wall):
+    t_min = 0.0
+    fl = 0.0
+    for ln, _ in per:
+        f64, f32 = launch_split_flops(info, ln)
+        t_min += max(f64 / PEAK_FP64 + f32 / PEAK_FP32, 8.0 * ln.nout / PEAK_HBM)
+        fl += f64 + f32
+    return {"roofline_time_s": t_min, "measured_s": wall, "frac": t_min / wall, "tflops": fl / wall / 1e12,
+            "note": "sum over launches of max(F64/78.6T + F32/157.3T, bytes/8TB/s) over the measured time"}
 
 
 def cpu_baseline(ss, seconds=12.0):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    """The oracle on every core this job may use, on uniform samples of the
+    screened set (ctypes releases the interpreter lock during each oracle
+    call, so the threads run in parallel)."""
     import oracle
     sa = oracle.ShellArrays(ss.shells)
     oracle.normalize(sa)
-    done = 0
-    t_used = 0.0
-    chunk = 200
-    seed = 0
-    while t_used < seconds:
-        qs = ss.sample(chunk, seed=seed)
-        seed += 1
-        t0 = time.perf_counter()
-        oracle.eri(sa, qs)
-        t_used += time.perf_counter() - t0
-        done += len(qs)
-        if t_used < seconds / 4:
-            chunk = min(chunk * 2, 20000)
-    return {"value": done / t_used, "unit": "shell quartets/s", "cores": 1, "kind": "port",
-            "sample": "%d quartets drawn uniformly from the screened 64-atom set, %.1f s of oracle time"
-                      % (done, t_used)}
+    cores = min(16, os.cpu_count() or 1)
+    oracle.eri(sa, ss.sample(8, seed=99))      # tables and plans built once
+    lock = threading.Lock()
+    tot = {"done": 0}
+    stop = time.perf_counter() + seconds
+
+    def work(t):
+        seed = 1000 * t
+        while time.perf_counter() < stop:
+            with lock:
+                qs = ss.sample(400, seed=seed)
+            seed += 1
+            oracle.eri(sa, qs)
+            with lock:
+                tot["done"] += len(qs)
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(t,)) for t in range(cores)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    el = time.perf_counter() - t0
+    model, flags = "unknown", ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("flags") and not flags:
+                flags = line
+    except OSError:
+        pass
+    isa = [x for x in ("avx512f", "avx2", "fma") if (" %s " % x) in flags + " "]
+    return {"value": tot["done"] / el, "unit": "shell quartets/s", "cores": cores, "kind": "port",
+            "cpu": model, "isa": isa,
+            "sample": "%d quartets drawn uniformly from the screened %d-atom set, %.1f s wall on %d threads "
+                      "of the oracle (gcc -O2 scalar C)" % (tot["done"], len(ss.shells) // 6, el, cores)}
+
+
+def serving_leg(nreq=10, rate=0.5, natom=(10, 48), seed=0):
+    """A bounded Poisson trace through the continuous-batching server."""
+    import serving
+    reqs = serving.request_stream(nreq, rate, seed, natom)
+    srv = serving.Server(hbm_budget=64 << 30, slots=4, tol=1e-10)
+    srv.serve(reqs)
+    res = serving.summarize(reqs, srv.wall)
+    res["max_concurrent"] = srv.max_concurrent
+    res["trace"] = ("%d requests, Poisson arrivals at %.2f/s, molecules of %d-%d atoms of the 64-atom recipe "
+                    "(own seeds), 30%% first derivatives, Schwarz 1e-10, 4 slots, 64 GiB admission budget"
+                    % (nreq, rate, natom[0], natom[1]))
+    res["latencies_s"] = [None if r.t_done is None else r.t_done - r.arrival for r in reqs]
+    return res
 
 
 def batched_leg(steps, warmup, nquartet=10 ** 6):
-    """The batched stream: 1e6 quartets over every class up to f (pool of
-    4096 shells, depths 1-6), one graph launch per pass of 256 quartet-list
-    class launches, inputs and the 5 GB output resident in HBM."""
+    """The batched stream: 1e6 quartets over every class up to f, one graph
+    launch per pass of 256 quartet-list class launches, inputs and the 5 GB
+    output resident in HBM."""
     import torch
     import batch
     import simint
@@ -120,11 +407,10 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
     progress("batched stream set up (%.1f s)" % setup)
     flops = 0.0
     for g in b.groups:
-        c = info["_".join("spdfghik"[x] for x in g.cls)]
+        c = info[cname(g.cls)]
         flops += g.prim_quartets * (c["f_setup"] + c["f_vrr"] + c["f_boys"] + c["bcont"]) + g.n * c["f_hrr"]
     out = torch.empty(b.out_doubles, dtype=torch.float64, device="cuda")
     stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
-    # one eager pass with per-class events (class breakdown)
     ev = []
 
     def rec(g, start):
@@ -133,12 +419,7 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
         ev.append((g, e))
     b.run(out.data_ptr(), events=rec)
     simint.synchronize()
-    progress("batched: eager pass with per-class events done")
-    per = []
-    for k in range(0, len(ev), 2):
-        g, e0 = ev[k]
-        _, e1 = ev[k + 1]
-        per.append((e0.elapsed_time(e1), g))
+    per = [(ev[k][1].elapsed_time(ev[k + 1][1]), ev[k][0]) for k in range(0, len(ev), 2)]
     gr = b.capture(out.data_ptr(), streams=BATCH_STREAMS)
     for _ in range(warmup):
         gr.launch()
@@ -165,8 +446,7 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
            "launch": "one hipGraph per pass holding %d quartet-list class launches on %d forked streams"
                      % (len(b.groups), BATCH_STREAMS),
            "setup_s": setup,
-           "top_classes": [{"class": "_".join("spdfghik"[x] for x in g.cls), "ms": ms, "quartets": g.n,
-                            "kernel": list_kernel_symbol(g.cls)}
+           "top_classes": [{"class": cname(g.cls), "ms": ms, "quartets": g.n, "kernel": list_kernel_symbol(g.cls)}
                            for ms, g in per[:8]]}
     fam = {}
     for ms, g in per:
@@ -175,86 +455,13 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
     res["eager_class_ms"] = {"total": sum(ms for ms, _ in per), "by_family": fam,
                              "note": "classes one after another on one stream, per-class events"}
     b.free()
+    basis.free()
     return res
-
-
-def mixed_leg(natom=64, threshold=1e-4):
-    """A system of the 64-atom recipe (64 atoms by default, 256 on request): every screened quartet
-    (Schwarz 1e-12) in FP64, then again with quartets whose bound is below
-    `threshold` run through the FP32-VRR/HRR kernels; one pass each (timed
-    on the library stream), then an untimed chunk-by-chunk comparison of the
-    two outputs for the max absolute error per class."""
-    import torch
-    import quartets
-    import simint
-    import workloads
-    shells = workloads.config2(natom=natom, seed=0)
-    t0 = time.perf_counter()
-    ss = quartets.ScreenedSet(shells, tol=1e-12)
-    plan64 = ss.launches
-    ss.mixed = threshold
-    ss._plan()
-    planm = ss.launches
-    setup = time.perf_counter() - t0
-    progress("%d-atom set: %d quartets (setup %.1f s)" % (natom, ss.total_quartets, setup))
-    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
-    ref = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
-    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
-
-    def timed(plan):
-        ss.launches = plan
-        ss.run_pass(out.data_ptr(), chunk=0)      # first use of the kernels
-        simint.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t = time.perf_counter()
-        e0.record(stream)
-        ss.run_pass(out.data_ptr())
-        e1.record(stream)
-        simint.synchronize()
-        return time.perf_counter() - t, e0.elapsed_time(e1) / 1000.0
-    w64, d64 = timed(plan64)
-    progress("FP64 pass %.1f s" % w64)
-    wm, dm = timed(planm)
-    progress("mixed pass %.1f s" % wm)
-    nmixed = sum(ln.nq for ln in planm if ln.precision == 1)
-    # error of the mixed pass against the FP64 pass, chunk by chunk (a
-    # chunk holds one class; FP64 launches of both plans are identical)
-    errs = {}
-    t_last = time.perf_counter()
-    ss.launches = planm
-    for c in range(ss.nchunk):
-        if time.perf_counter() - t_last > 20.0:
-            progress("error check: chunk %d of %d" % (c, ss.nchunk))
-            t_last = time.perf_counter()
-        if not any(ln.precision == 1 for ln in ss.chunk_launches(c)):
-            continue
-        n = ss.chunk_doubles(c)
-        ss.launches = plan64
-        ss.run_pass(ref.data_ptr(), chunk=c)
-        ss.launches = planm
-        ss.run_pass(out.data_ptr(), chunk=c)
-        simint.synchronize()
-        e = float((out[:n] - ref[:n]).abs().max())
-        cls = ss.chunk_launches(c)[0].cls
-        name = "_".join("spdfghik"[x] for x in cls)
-        errs[name] = max(errs.get(name, 0.0), e)
-    ss.launches = planm
-    ss.free()
-    return {"workload": "%d-atom grid of the 64-atom recipe (%d shells), Schwarz 1e-12; quartets with "
-                        "sqrt((ab|ab)(cd|cd)) < %g (scaled per class by its HRR sides) run the FP32 VRR"
-                        % (natom, len(shells), threshold),
-            "quartets": ss.total_quartets, "fp32_quartets": nmixed,
-            "mixed_value": ss.total_quartets / wm, "fp64_value": ss.total_quartets / w64,
-            "speedup": w64 / wm, "mixed_device_s": dm, "fp64_device_s": d64, "unit": "shell quartets/s",
-            "max_abs_err_vs_fp64": max(errs.values()) if errs else 0.0,
-            "max_abs_err_per_class": errs, "bound": 1e-10, "setup_s": setup}
 
 
 def deriv_leg(natom=64, tol=1e-12):
     """First derivatives of every screened quartet of the 64-atom system
-    (12 values per Cartesian), one pass through a 1 GiB chunk buffer on the
-    device (simint_hip_compute_eri_deriv1_range per launch)."""
+    (12 values per Cartesian), one pass through a 1 GiB chunk buffer."""
     import torch
     import quartets
     import simint
@@ -287,23 +494,44 @@ def deriv_leg(natom=64, tol=1e-12):
     return res
 
 
+def fp64_64atom_leg():
+    """The 64-atom all-classes-to-d system in plain FP64, one full pass."""
+    import torch
+    import quartets
+    import simint
+    import workloads
+    ss = quartets.ScreenedSet(workloads.config2(natom=64, seed=0), tol=1e-12)
+    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+    ss.run_launches(out.data_ptr(), shares(ss.launches, 4)[0])
+    simint.synchronize()
+    t = time.perf_counter()
+    ss.run_pass(out.data_ptr())
+    simint.synchronize()
+    wall = time.perf_counter() - t
+    res = {"workload": "64-atom jittered 3-bohr grid, s6 s3 p3 s1 p1 d1 per atom, every unique quartet with "
+                       "Schwarz bound >= 1e-12, plain FP64", "quartets": ss.total_quartets,
+           "value": ss.total_quartets / wall, "unit": "shell quartets/s", "pass_s": wall,
+           "tflops": ss.total_flops / wall / 1e12}
+    ss.free()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--natom", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=3, help="shares of the quartet set timed (together one pass)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shares run first")
+    ap.add_argument("--natom", type=int, default=256)
+    ap.add_argument("--mixed-threshold", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched all-classes stream")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving trace")
     ap.add_argument("--no-deriv", action="store_true", help="skip the first-derivative leg")
-    ap.add_argument("--mixed-natom", type=int, default=64,
-                    help="atoms of the mixed-precision leg (the 256-atom system takes ~7 minutes)")
+    ap.add_argument("--no-fp64-64", action="store_true", help="skip the 64-atom FP64 leg")
+    ap.add_argument("--no-errcheck", action="store_true", help="profiling aid: skip the error measurement")
     ap.add_argument("--tuning", default=None, help="apply this kernel-family tuning file before running")
     ap.add_argument("--only-class", default=None,
                     help="profiling aid: run only this class's launches (e.g. d_p_d_p)")
-    ap.add_argument("--force-family", choices=["lpq", "wpq", "lpqs_a", "lpqs_b"], default=None,
-                    help="profiling aid: use this kernel family for every class that has it")
     ap.add_argument("--copy-gib", type=float, default=None,
                     help="profiling aid: run one device copy of this many GiB and exit "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -325,97 +553,22 @@ def main():
         print(json.dumps({"copy_bytes": 8 * n}))
         return
     sys.path.insert(0, PKG)
-    import quartets
     import simint
-    import workloads
-
-    shells = workloads.config2(natom=args.natom, seed=0)
     if args.tuning:
         simint.apply_tuning(args.tuning)
-    if args.force_family:
-        fam = simint.FAMILIES[args.force_family]
-        for a in range(3):
-            for b in range(3):
-                for c in range(3):
-                    for d in range(3):
-                        simint.lib().simint_hip_set_class_kernel(a, b, c, d, fam)
-    t_setup = time.perf_counter()
-    ss = quartets.ScreenedSet(shells, tol=1e-12)
-    t_setup = time.perf_counter() - t_setup
-    progress("64-atom set: %d quartets in %d launches (setup %.1f s)" % (ss.total_quartets, len(ss.launches),
-                                                                         t_setup))
-    if args.only_class:
-        ss.launches = [ln for ln in ss.launches if quartets.simint_class_name(ln.cls) == args.only_class]
-        ss.total_quartets = sum(ln.nq for ln in ss.launches)
-        ss.total_flops = sum(ln.flops for ln in ss.launches)
-        ss.total_bytes = sum(8.0 * ln.nout for ln in ss.launches)
-    out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
-    stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
 
-    for _ in range(args.warmup):
-        ss.run_pass(out.data_ptr())
-    simint.synchronize()
-    torch.cuda.synchronize()
-    progress("warmup done")
-
-    events = []
-
-    def rec(ln, start):
-        e = torch.cuda.Event(enable_timing=True)
-        e.record(stream)
-        events.append((ln, start, e))
-
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ss.run_pass(out.data_ptr(), events=rec)
-    simint.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
-    progress("timed steps done: %.1f ms per step" % ms_per_step)
-    value = ss.total_quartets * args.steps / elapsed
-
-    # per-class device time from the events
-    per = {}
-    for k in range(0, len(events), 2):
-        ln, _, e0 = events[k]
-        _, _, e1 = events[k + 1]
-        ms = e0.elapsed_time(e1)
-        c = per.setdefault(ln.cls, {"ms": 0.0, "flops": 0.0, "bytes": 0.0, "launches": 0, "quartets": 0})
-        c["ms"] += ms
-        c["flops"] += ln.flops
-        c["bytes"] += 8.0 * ln.nout
-        c["launches"] += 1
-        c["quartets"] += ln.nq
-    dom = max(per, key=lambda k: per[k]["ms"])
-    d = per[dom]
-    mean_s = d["ms"] / 1000.0 / d["launches"]
-    flops_per_launch = d["flops"] / d["launches"]
-    bytes_per_launch = d["bytes"] / d["launches"]
-    t_flop = flops_per_launch / PEAK_FP64
-    t_byte = bytes_per_launch / PEAK_HBM
-    if t_flop >= t_byte:
-        achieved = flops_per_launch / mean_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / (PEAK_FP64 / 1e12), "traffic": None}
-    else:
-        achieved = bytes_per_launch / mean_s / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": achieved / (PEAK_HBM / 1e9), "traffic": None}
-    kname = kernel_symbol(ss.info, dom)
-    roof["traffic"] = measured_traffic(kname)
-    roof.update({"kernel": kname,
-                 "algorithmic_bytes_per_launch": bytes_per_launch,
-                 "compute": "FP64 VALU (no MFMA); peak is the FP64 vector rate",
-                 "launches": d["launches"], "mean_launch_ms": 1000.0 * mean_s,
-                 "flops_per_launch": flops_per_launch, "share_of_device_time":
-                     d["ms"] / sum(v["ms"] for v in per.values())})
-    whole = {"tflops": ss.total_flops * args.steps / elapsed / 1e12,
-             "frac_fp64_peak": ss.total_flops * args.steps / elapsed / PEAK_FP64,
-             "output_GBps": ss.total_bytes * args.steps / elapsed / 1e9}
-
+    m = mixed_system_leg(args)
+    info = m["info"]
+    value = m["quartets"] / m["wall_mixed"]
+    fp64_value = m["quartets"] / m["wall_fp64"]
+    roof, agg = roofline_of(m["per_launch"], info)
+    per_class = {}
+    for (cls, prec), a in agg.items():
+        pc = per_class.setdefault(cname(cls), {"ms_mixed": 0.0})
+        pc["ms_mixed"] += 1000.0 * a["s"]
+    for ln, s in m["per_launch64"]:
+        pc = per_class.setdefault(cname(ln.cls), {"ms_mixed": 0.0})
+        pc["ms_fp64"] = pc.get("ms_fp64", 0.0) + 1000.0 * s
     line = {
         "metric": METRIC,
         "value": value,
@@ -423,53 +576,60 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1000.0 * m["wall_mixed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "%d-atom jittered 3-bohr grid, s6 s3 p3 s1 p1 d1 per atom, all unique "
-                               "quartets with Schwarz bound >= 1e-12, plain FP64 (l <= 2)" % args.natom,
-                   "natom": args.natom, "nshell": len(shells), "quartets_per_step": ss.total_quartets,
-                   "launches_per_step": len(ss.launches), "path": "device-resident batched",
-                   "setup_s": t_setup},
+        "config": {"workload": "%d-atom jittered 3-bohr grid (8x8x4), s6 s3 p3 s1 p1 d1 per atom, every unique "
+                               "quartet with Schwarz bound >= 1e-12, mixed precision: FP32 VRR for quartets with "
+                               "bound < %g (/400 with one HRR side, /1e5 with two), Boys, contraction and HRR "
+                               "FP64" % (args.natom, args.mixed_threshold),
+                   "natom": args.natom, "nshell": 6 * args.natom, "quartets_per_pass": m["quartets"],
+                   "fp32_vrr_quartets": m["fp32_quartets"], "step": "1/%d of the quartet set" % args.steps,
+                   "path": "device-resident screened ranges (simint_hip_compute_eri_range)",
+                   "setup_s": m["setup_s"]},
+        "mixed": {"value": value, "device_s": m["dev_mixed"], "wall_s": m["wall_mixed"]},
+        "fp64": {"value": fp64_value, "device_s": m["dev_fp64"], "wall_s": m["wall_fp64"],
+                 "ms_per_step": 1000.0 * m["wall_fp64"] / args.steps},
+        "mixed_over_fp64": value / fp64_value,
         "roofline": roof,
-        "whole_job": whole,
+        "whole_job": whole_job(m["per_launch"], info, m["wall_mixed"]),
+        "whole_job_fp64": whole_job(m["per_launch64"], info, m["wall_fp64"]),
     }
-    if not args.no_batched and not args.only_class:
-        ss.free()
-        line["batched"] = batched_leg(args.steps, args.warmup)
-        progress("batched leg done")
-    if not args.no_mixed and not args.only_class:
-        line["mixed"] = mixed_leg(args.mixed_natom)
-        progress("mixed leg done")
-    if not args.no_deriv and not args.only_class:
-        line["deriv1"] = deriv_leg()
-        progress("deriv leg done")
+    if "dev_err" in m:
+        line["error"] = {"bound": 1e-10,
+                         "max_abs_err_vs_oracle": max(m["orc_err"].values()) if m["orc_err"] else 0.0,
+                         "max_abs_err_vs_fp64_gpu": max(m["dev_err"].values()) if m["dev_err"] else 0.0,
+                         "per_class_vs_oracle": m["orc_err"], "per_class_vs_fp64_gpu": m["dev_err"],
+                         "sample": m["err_cover"],
+                         "note": "classes absent here run FP64 throughout (no quartet under the threshold, or "
+                                 "their kernel has no FP32 variant)"}
+    top = sorted(per_class.items(), key=lambda kv: -kv[1]["ms_mixed"])[:10]
+    line["top_classes"] = [dict(cls=k, **v) for k, v in top]
+    ss = m["ss"]
+    if not args.no_serving and not args.only_class:
+        progress("serving trace")
+        line["serving"] = serving_leg()
+        line["p50_latency_s"] = line["serving"]["p50_latency_s"]
+        line["p99_latency_s"] = line["serving"]["p99_latency_s"]
+        progress("serving leg done")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ss)
         progress("cpu baseline done")
-    # device time by (class, primitive quartets per quartet)
-    perk = {}
-    for k in range(0, len(events), 2):
-        ln, _, e0 = events[k]
-        _, _, e1 = events[k + 1]
-        key = (ln.cls, int(ln.bra.nprim * ln.ket.nprim))
-        c = perk.setdefault(key, [0.0, 0])
-        c[0] += e0.elapsed_time(e1)
-        c[1] += ln.nq
-    line["top_class_K"] = [{"class": quartets.simint_class_name(k[0]), "K": k[1], "ms_per_step": v[0] / args.steps,
-                            "quartets_per_step": v[1] // args.steps}
-                           for k, v in sorted(perk.items(), key=lambda kv: -kv[1][0])[:12]]
-    top = sorted(per.items(), key=lambda kv: -kv[1]["ms"])[:8]
-    line["top_classes"] = [{"class": quartets.simint_class_name(k), "ms_per_step": v["ms"] / args.steps,
-                            "tflops": v["flops"] / (v["ms"] / 1000.0) / 1e12,
-                            "quartets_per_step": v["quartets"] // args.steps,
-                            "kernel": kernel_symbol(ss.info, k)} for k, v in top]
+    ss.free()
+    del m
+    if not args.no_batched and not args.only_class:
+        line["batched"] = batched_leg(3, 1)
+        progress("batched leg done")
+    if not args.no_deriv and not args.only_class:
+        line["deriv1"] = deriv_leg()
+        progress("deriv leg done")
+    if not args.no_fp64_64 and not args.only_class:
+        line["fp64_64atom"] = fp64_64atom_leg()
+        progress("64-atom FP64 leg done")
     print(json.dumps(line))
-    if args.no_batched or args.only_class:
-        ss.free()
 
 
 if __name__ == "__main__":

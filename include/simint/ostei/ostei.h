@@ -9,6 +9,16 @@
 extern "C" {
 #endif
 
+/* Pointer to a class function of the reference's dispatch table
+ * (skel/simint/ostei/ostei.h:11-15): shell pairs passed by value, screening
+ * tolerance, workspace, output.  Kept for source compatibility; this library
+ * dispatches to device kernels internally and exports no table of these. */
+typedef int (*simint_osteifunc)(struct simint_multi_shellpair const,
+                                struct simint_multi_shellpair const,
+                                double,
+                                double * restrict,
+                                double * restrict);
+
 int simint_compute_ostei(struct simint_multi_shellpair const * P,
                          struct simint_multi_shellpair const * Q,
                          double screen_tol,

@@ -86,6 +86,21 @@ long long simint_hip_compute_eri_range(simint_hip_pairs const * P, simint_hip_pa
 /* (bb|bb) for every pair b of P, at d_out + b * ncart(abab). */
 long long simint_hip_compute_eri_diag(simint_hip_pairs const * P, double * d_out);
 
+/* One-electron integrals of many shell pairs (no counterpart in the
+ * reference, whose osoei functions take one pair per call).  A shell set is
+ * copied to the device once; pair i is shells d_pairs[2i] and d_pairs[2i+1]
+ * of the set, its ncart(a) x ncart(b) values (Psi4 order) written at
+ * d_out + d_off[i].  kind 0: overlap, 1: kinetic energy, 2: nuclear
+ * attraction -sum_C Z_C <a|1/r_C|b> of the ncenter point charges in
+ * d_charges (device, one row of Z, x, y, z per charge).  Angular momentum up
+ * to 5.  Queues the kernel on the calling thread's stream; returns npair or
+ * a negative SIMINT_ERROR_*. */
+typedef struct simint_hip_shells simint_hip_shells;
+int  simint_hip_shells_create(int nshell, struct simint_shell const * shells, simint_hip_shells ** out);
+void simint_hip_shells_free(simint_hip_shells * s);
+long long simint_hip_compute_osoei(int kind, simint_hip_shells const * S, long long npair, int const * d_pairs,
+                                   long long const * d_off, int ncenter, double const * d_charges, double * d_out);
+
 int   simint_hip_synchronize(void);
 void *simint_hip_stream(void);                /* hipStream_t of the calling thread */
 /* Queue the calling thread's later calls on `stream` (a hipStream_t; NULL:

@@ -24,6 +24,7 @@
 #include "simint/ostei/ostei.h"
 #include "device_types.h"
 #include "dispatch.h"
+#include "runtime_internal.h"
 
 namespace {
 
@@ -45,6 +46,7 @@ constexpr int D1_N = SIMINT_OSTEI_DERIV1_MAXAM + 1;
 struct DeviceState {
     std::once_flag once;
     bool ok = false;
+    std::atomic<bool> ready{false};   // set once the device is fully set up
     int id = 0;
     double *boys = nullptr;
     const D1Entry *d1tab[D1_N][D1_N][D1_N][D1_N] = {};
@@ -140,6 +142,7 @@ void device_init()
     g_dev.id = id;
     g_dev.boys = d;
     g_dev.ok = true;
+    g_dev.ready.store(true, std::memory_order_release);
 }
 
 bool ensure_device()
@@ -1094,14 +1097,15 @@ long long simint_hip_deriv1_worksize(const simint_hip_pairs *P, const simint_hip
     return (long long)deriv1_plan(am).scratch * nquartet;
 }
 
-long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const simint_hip_pairs *Q,
-                                             long long nquartet, const int *d_qbra, const int *d_qket,
-                                             const long long *d_qout, double screen_tol, double *d_work,
-                                             double *d_out)
+}  // extern "C"
+
+namespace {
+
+// the derivative list with its class blocks in d_work (a caller's buffer or
+// the fixed scratch area); the device is set up
+long long deriv1_list(const simint_hip_pairs *P, const simint_hip_pairs *Q, long long nquartet, const int *d_qbra,
+                      const int *d_qket, const long long *d_qout, double screen_tol, double *d_work, double *d_out)
 {
-    if (!P || !Q || nquartet < 0 || (nquartet > 0 && (!d_qbra || !d_qket || !d_out || !d_work)))
-        return SIMINT_ERROR_ARGUMENT;
-    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
     if (!deriv1_built(P->am1, P->am2, Q->am1, Q->am2)) {
         t_err = "first derivatives not built for this class";
         return SIMINT_ERROR_UNSUPPORTED;
@@ -1147,6 +1151,21 @@ long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const si
     d.qout = d_qout;
     if ((e = launch_deriv1_assemble(d, c.s)) != hipSuccess) { set_err("deriv1 assembly", e); return SIMINT_ERROR_HIP; }
     return nquartet;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const simint_hip_pairs *Q,
+                                             long long nquartet, const int *d_qbra, const int *d_qket,
+                                             const long long *d_qout, double screen_tol, double *d_work,
+                                             double *d_out)
+{
+    if (!P || !Q || nquartet < 0 || (nquartet > 0 && (!d_qbra || !d_qket || !d_out || !d_work)))
+        return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    return deriv1_list(P, Q, nquartet, d_qbra, d_qket, d_qout, screen_tol, d_work, d_out);
 }
 
 long long simint_hip_compute_eri_deriv1_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0,
@@ -1287,3 +1306,33 @@ int simint_hip_set_stream(void *stream)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ services for other sources
+namespace simint_rt {
+
+bool ensure_device() { return ::ensure_device(); }
+
+const double *boys_grid() { return g_dev.boys; }
+
+hipStream_t stream() { return ctx().s; }
+
+hipError_t thread_buffers(size_t host_bytes, size_t in_bytes, size_t out_bytes, void **host, void **din,
+                          void **dout)
+{
+    ThreadCtx &c = ctx();
+    hipError_t e;
+    if ((e = hipStreamSynchronize(c.s)) != hipSuccess) return e;
+    if ((e = c.stage.reserve(host_bytes)) != hipSuccess) return e;
+    if ((e = c.in.reserve(in_bytes)) != hipSuccess) return e;
+    if ((e = c.out.reserve(out_bytes)) != hipSuccess) return e;
+    *host = c.stage.p;
+    *din = c.in.p;
+    *dout = c.out.p;
+    return hipSuccess;
+}
+
+void set_error(const char *msg) { t_err = msg; }
+
+void set_hip_error(const char *what, hipError_t e) { set_err(what, e); }
+
+}  // namespace simint_rt

@@ -241,10 +241,15 @@ class ScreenedSet:
         library stream.  If events is a callable, it is called before and
         after each launch with the launch (to record timing events on the
         library stream)."""
+        self.run_launches(out_ptr, self.launches if chunk is None else self.chunk_launches(chunk), events,
+                          work_ptr)
+
+    def run_launches(self, out_ptr, launches, events=None, work_ptr=None):
+        """Queue the given launches of this set's plans (as run_pass)."""
         self.work_ptr = work_ptr
         if self.deriv and not work_ptr:
             raise ValueError("a derivative pass needs a work buffer (deriv_worksize doubles)")
-        for ln in (self.launches if chunk is None else self.chunk_launches(chunk)):
+        for ln in launches:
             if events:
                 events(ln, True)
             if self.deriv:
@@ -276,6 +281,32 @@ class ScreenedSet:
             bp = ln.bra.pairs[ln.b0 + bi]
             kp = ln.ket.pairs[k]
             out.append((bp[0], bp[1], kp[0], kp[1]))
+        return out
+
+    def located_sample(self, n, seed=0, precision=None, cls=None, launches=None):
+        """n random quartets of the screened set (or of the given launches)
+        with where a pass writes them: a list of (chunk, offset in the chunk
+        buffer, doubles, (i, j, k, l)), uniform over the quartets of the
+        launches of one precision and one class if given."""
+        rng = np.random.default_rng(seed)
+        pool = self.launches if launches is None else launches
+        w = np.array([ln.nq if (precision is None or ln.precision == precision) and (cls is None or ln.cls == cls)
+                      else 0 for ln in pool], dtype=np.float64)
+        if not len(w) or w.sum() == 0:
+            return []
+        out = []
+        for li in rng.choice(len(pool), size=n, p=w / w.sum()):
+            ln = pool[li]
+            lo = ln.lo if ln.lo is not None else np.zeros_like(ln.nket)
+            cum = np.cumsum(ln.nket.astype(np.int64) - lo)
+            r = int(rng.integers(0, cum[-1]))
+            bi = int(np.searchsorted(cum, r, side="right"))
+            j = r - (int(cum[bi - 1]) if bi > 0 else 0)
+            k = int(lo[bi]) + j
+            nc = ln.nout // ln.nq
+            bp = ln.bra.pairs[ln.b0 + bi]
+            kp = ln.ket.pairs[k]
+            out.append((ln.chunk, int(ln.off[bi]) + j * nc, nc, (bp[0], bp[1], kp[0], kp[1])))
         return out
 
     def free(self):

@@ -46,6 +46,7 @@ class Request:
         self.t_admit = self.t_done = None
         self.quartets = 0
         self.digest = None
+        self.samples = None        # [(quartet, values)] when the server probes results
         self.error = None
 
     def to_dict(self):
@@ -82,8 +83,9 @@ def footprint(natom, kind, chunk_doubles):
 
 
 class Server:
-    def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False):
+    def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False, probe=0):
         self.verbose = verbose
+        self.probe = probe         # quartets per request whose values are kept (result checks)
         self.budget = float(hbm_budget)
         self.free = float(hbm_budget)
         self.slots = slots
@@ -150,6 +152,11 @@ class Server:
         if deriv:
             work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
         t_note = time.perf_counter()
+        probes = {}
+        if self.probe:
+            for c, off, nc, q in ss.located_sample(self.probe, seed=r.seed):
+                probes.setdefault(c, []).append((off, nc, q))
+        kept = []
         with torch.cuda.stream(stream):
             # zeroed on the library stream, so the fill is ordered before the
             # accumulations that stream runs
@@ -164,9 +171,13 @@ class Server:
                 x = out[:n]
                 s[0] += x.sum()
                 s[1] += torch.dot(x, x)
+                for off, nc, q in probes.get(c, []):
+                    kept.append((q, out[off:off + nc].clone()))
             stream.synchronize()
         r.quartets = ss.total_quartets
         r.digest = [float(v) for v in s.cpu()]
+        if self.probe:
+            r.samples = [(q, v.cpu().numpy()) for q, v in kept]
         ss.free()
 
     # ------------------------------------------------------------------ run

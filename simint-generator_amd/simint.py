@@ -61,6 +61,9 @@ EXPORTS = [
     "simint_hip_set_list_split",
     "simint_hip_set_class_list_kernel", "simint_hip_compute_eri_deriv1_list",
     "simint_hip_capture_begin", "simint_hip_capture_end", "simint_hip_graph_launch", "simint_hip_graph_free",
+    "simint_compute_overlap", "simint_compute_ke", "simint_compute_potential",
+    "simint_compute_osoei_overlap", "simint_compute_osoei_ke", "simint_compute_osoei_potential",
+    "simint_hip_shells_create", "simint_hip_shells_free", "simint_hip_compute_osoei",
 ]
 
 _LIB = None
@@ -125,6 +128,15 @@ def lib():
         L.simint_hip_set_class_kernel.argtypes = [c_int] * 5
         L.simint_hip_class_list_kernel.argtypes = [c_int] * 4
         L.simint_hip_set_class_list_kernel.argtypes = [c_int] * 5
+        for f in ("simint_compute_overlap", "simint_compute_ke", "simint_compute_osoei_overlap",
+                  "simint_compute_osoei_ke"):
+            getattr(L, f).argtypes = [SP, SP, dptr]
+        for f in ("simint_compute_potential", "simint_compute_osoei_potential"):
+            getattr(L, f).argtypes = [c_int, dptr, dptr, dptr, dptr, SP, SP, dptr]
+        L.simint_hip_shells_create.argtypes = [c_int, SP, ctypes.POINTER(c_void_p)]
+        L.simint_hip_shells_free.argtypes = [c_void_p]
+        L.simint_hip_compute_osoei.argtypes = [c_int, c_void_p, ll, c_void_p, c_void_p, c_int, c_void_p, c_void_p]
+        L.simint_hip_compute_osoei.restype = ll
         _LIB = L
     return _LIB
 
@@ -377,6 +389,54 @@ def compute_diag(P, d_out_ptr):
     if r < 0:
         raise RuntimeError("simint_hip_compute_eri_diag failed (%d): %s" % (r, last_error()))
     return r
+
+
+OEI_KINDS = {"overlap": 0, "ke": 1, "potential": 2}
+
+
+def compute_oei(kind, basis, i, j, charges=None):
+    """Drop-in one-electron call on shells i and j of `basis`:
+    simint_compute_overlap / _ke / _potential; charges = (Z, xyz) arrays for
+    the potential.  Returns (rc, ncart(a) x ncart(b) values, flat)."""
+    n = ncart(basis.arr[i].am) * ncart(basis.arr[j].am)
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    A, B = ctypes.byref(basis.arr[i]), ctypes.byref(basis.arr[j])
+    if kind == "potential":
+        Z = np.ascontiguousarray(charges[0], dtype=np.float64)
+        C = np.ascontiguousarray(charges[1], dtype=np.float64).reshape(-1, 3)
+        xs, ys, zs = (np.ascontiguousarray(C[:, k]) for k in range(3))
+        rc = lib().simint_compute_potential(len(Z), Z.ctypes.data_as(dptr), xs.ctypes.data_as(dptr),
+                                            ys.ctypes.data_as(dptr), zs.ctypes.data_as(dptr), A, B,
+                                            out.ctypes.data_as(dptr))
+    else:
+        fn = lib().simint_compute_overlap if kind == "overlap" else lib().simint_compute_ke
+        rc = fn(A, B, out.ctypes.data_as(dptr))
+    return rc, out[:n]
+
+
+class DeviceShells:
+    """simint_hip_shells: a shell set resident in HBM for batched
+    one-electron integrals."""
+
+    def __init__(self, basis):
+        h = c_void_p()
+        rc = lib().simint_hip_shells_create(basis.n, basis.arr, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError("simint_hip_shells_create failed (%d): %s" % (rc, last_error()))
+        self.h = h
+
+    def compute(self, kind, npair, d_pairs, d_off, d_out, ncenter=0, d_charges=0):
+        r = lib().simint_hip_compute_osoei(OEI_KINDS[kind], self.h, int(npair), c_void_p(d_pairs),
+                                           c_void_p(d_off), int(ncenter), c_void_p(d_charges or None),
+                                           c_void_p(d_out))
+        if r < 0:
+            raise RuntimeError("simint_hip_compute_osoei failed (%d): %s" % (r, last_error()))
+        return r
+
+    def free(self):
+        if self.h:
+            lib().simint_hip_shells_free(self.h)
+            self.h = None
 
 
 def synchronize():

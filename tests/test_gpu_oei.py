@@ -1,0 +1,137 @@
+"""One-electron integrals on the GPU (simint_compute_overlap / _ke /
+_potential and the batched simint_hip_compute_osoei) against the oracle's
+restatement of skel/simint/osoei and its long-double Hermite expansions.
+
+Bar: per shell pair, max|gpu - oracle| <= 1e-14 max|oracle| (normwise); the
+nuclear attraction additionally within the Boys scheme's own error of the
+exact long-double values."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import simint
+from helpers import normalized_arrays
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-14
+
+
+def _shells(seed, lmax=3, per_l=2, depth=(1, 5), lo=0.05, hi=80.0, box=2.5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for l in range(lmax + 1):
+        for _ in range(per_l):
+            n = int(rng.integers(depth[0], depth[1] + 1))
+            out.append((l, tuple(rng.uniform(-box, box, 3)), list(np.exp(rng.uniform(np.log(lo), np.log(hi), n))),
+                        list(rng.uniform(0.1, 1.0, n))))
+    return out
+
+
+def _worst(a, b, off, n):
+    return max(np.abs(a[o:o + m] - b[o:o + m]).max() / max(np.abs(b[o:o + m]).max(), 1e-300)
+               for o, m in zip(off, n))
+
+
+CHARGES = (np.array([1.0, 6.0, 8.0, 92.0]), np.random.default_rng(7).uniform(-2.5, 2.5, (4, 3)))
+
+
+@pytest.mark.parametrize("kind", ["overlap", "ke", "potential"])
+def test_dropin_oei_vs_oracle(gpu, kind):
+    """Every ordered pair of 8 shells (s..f, depths 1-5), one drop-in call
+    each, against the oracle; the potential also against the exact values."""
+    shells = _shells(11)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    pairs = [(i, j) for i in range(len(shells)) for j in range(len(shells))]
+    k = simint.OEI_KINDS[kind]
+    Z, C = CHARGES
+    ref, off, n = oracle.osoei(sa, pairs, k, Z if k == 2 else None, C if k == 2 else None)
+    got = np.zeros_like(ref)
+    for (i, j), o, m in zip(pairs, off, n):
+        rc, v = simint.compute_oei(kind, basis, i, j, CHARGES)
+        assert rc == 1, (rc, simint.last_error())
+        got[o:o + m] = v
+    w = _worst(got, ref, off, n)
+    print(kind, "worst normwise vs oracle", w)
+    assert w <= TOL, w
+    if k == 2:
+        ex, _, _ = oracle.md_oei(sa, pairs, 2, Z, C)
+        eo = _worst(ref, ex, off, n)
+        eg = _worst(got, ex, off, n)
+        assert eg <= max(TOL, 4 * eo), (eg, eo)
+    basis.free()
+
+
+def test_batched_oei_g_h_shells(gpu):
+    """simint_hip_compute_osoei over all pairs of a set with g and h shells
+    (the 57 KB-LDS instantiation), pairs written at arbitrary offsets."""
+    shells = _shells(5, lmax=5, per_l=2, depth=(1, 4))
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    pairs = [(i, j) for i in range(len(shells)) for j in range(len(shells))]
+    Z, C = CHARGES
+    ds = simint.DeviceShells(basis)
+    try:
+        for kind, k in simint.OEI_KINDS.items():
+            ref, off, n = oracle.osoei(sa, pairs, k, Z if k == 2 else None, C if k == 2 else None)
+            # reversed placement with gaps: checks the offsets are honoured
+            gap = 3
+            offs = np.zeros(len(pairs), dtype=np.int64)
+            pos = 0
+            for q in reversed(range(len(pairs))):
+                offs[q] = pos
+                pos += int(n[q]) + gap
+            d_pairs = torch.tensor(np.array(pairs, dtype=np.int32).ravel(), device="cuda")
+            d_off = torch.tensor(offs, device="cuda")
+            ch = torch.tensor(np.column_stack([Z, C]).ravel(), dtype=torch.float64, device="cuda")
+            out = torch.full((pos,), float("nan"), dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            ds.compute(kind, len(pairs), d_pairs.data_ptr(), d_off.data_ptr(), out.data_ptr(),
+                       len(Z) if k == 2 else 0, ch.data_ptr())
+            simint.synchronize()
+            h = out.cpu().numpy()
+            got = np.concatenate([h[offs[q]:offs[q] + n[q]] for q in range(len(pairs))])
+            w = _worst(got, ref, off, n)
+            print(kind, "g/h batch worst normwise vs oracle", w)
+            assert w <= TOL, (kind, w)
+    finally:
+        ds.free()
+        basis.free()
+
+
+def test_oei_s_closed_forms(gpu):
+    """s-type closed forms on the GPU: S = (pi/p)^(3/2) e^{-mu R^2},
+    T = mu (3 - 2 mu R^2) S, V = -Z 2 pi / p e^{-mu R^2} F_0(p |P - C|^2)."""
+    from math import erf, exp, pi, sqrt
+    a, b = 0.7, 1.9
+    A, B, Cc = np.array([0.1, -0.4, 0.3]), np.array([-0.5, 0.6, 1.1]), np.array([0.9, 0.2, -0.7])
+    basis = simint.Basis([(0, tuple(A), [a], [1.0]), (0, tuple(B), [b], [1.0])], normalize=False)
+    p, mu = a + b, a * b / (a + b)
+    R2 = float(((A - B) ** 2).sum())
+    S = (pi / p) ** 1.5 * exp(-mu * R2)
+    T = mu * (3.0 - 2.0 * mu * R2) * S
+    P = (a * A + b * B) / p
+    x = p * float(((P - Cc) ** 2).sum())
+    V = -3.0 * 2.0 * pi / p * exp(-mu * R2) * 0.5 * sqrt(pi / x) * erf(sqrt(x))
+    for kind, e in (("overlap", S), ("ke", T), ("potential", V)):
+        rc, v = simint.compute_oei(kind, basis, 0, 1, (np.array([3.0]), Cc.reshape(1, 3)))
+        assert rc == 1
+        assert abs(v[0] - e) <= 1e-14 * abs(e), (kind, v[0], e)
+    basis.free()
+
+
+def test_oei_limits(gpu):
+    """AM above 5 is refused with SIMINT_ERROR_UNSUPPORTED before any launch;
+    a potential with no charges is zero."""
+    basis = simint.Basis([(6, (0.0, 0.0, 0.0), [1.0], [1.0]), (0, (0.0, 0.0, 1.0), [1.0], [1.0])])
+    out = np.zeros(28, dtype=np.float64)
+    rc = simint.lib().simint_compute_overlap(ctypes.byref(basis.arr[0]), ctypes.byref(basis.arr[1]),
+                                             out.ctypes.data_as(simint.dptr))
+    assert rc == simint.ERROR_UNSUPPORTED
+    rc, v = simint.compute_oei("potential", basis, 1, 1, (np.zeros(0), np.zeros((0, 3))))
+    assert rc == 1 and v[0] == 0.0
+    basis.free()

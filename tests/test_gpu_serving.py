@@ -1,7 +1,8 @@
-"""The serving loop: requests of different sizes and kinds admitted while
+"""The serving loop: requests of 10-20 atoms of both kinds admitted while
 others run, overlapping on the device; each result (an FP64 digest of every
-integral of the request) is the same alone and in the mix; first-derivative
-passes over a screened set match the oracle."""
+integral of the request) is the same alone and in the mix; the integrals
+served in the mix match the oracle on a sample of every request's quartets;
+first-derivative passes over a screened set match the oracle."""
 import numpy as np
 import pytest
 
@@ -10,13 +11,13 @@ import simint
 import quartets
 import serving
 import workloads
-from helpers import normalized_arrays, parity_report_deriv
+from helpers import normalized_arrays, parity_report, parity_report_deriv
 
 pytestmark = pytest.mark.gpu
 
 
 def _reqs():
-    spec = [(6, 11, "eri"), (9, 12, "deriv"), (4, 13, "eri"), (12, 14, "eri"), (5, 15, "deriv"), (8, 16, "eri")]
+    spec = [(10, 11, "eri"), (12, 12, "deriv"), (10, 13, "eri"), (20, 14, "eri"), (10, 15, "deriv"), (14, 16, "eri")]
     return [serving.Request(i, n, s, k, 0.0) for i, (n, s, k) in enumerate(spec)]
 
 
@@ -38,6 +39,33 @@ def test_same_alone_and_in_mix(gpu):
     for r in mix:
         assert r.error is None, r.error
         assert (r.digest, r.quartets) == alone[r.rid], r.rid
+
+
+def test_served_results_vs_oracle(gpu):
+    """Two ERI requests and two first-derivative requests of 10-16 atoms,
+    served concurrently; 48 quartets of each, read from the output chunks
+    the server streams, against the oracle (ERIs: the parity bar with exact
+    values; derivatives: 1e-10 absolute and as accurate as the oracle)."""
+    spec = [(16, 41, "eri"), (10, 42, "deriv"), (12, 43, "eri"), (11, 44, "deriv")]
+    reqs = [serving.Request(i, n, s, k, 0.02 * i) for i, (n, s, k) in enumerate(spec)]
+    srv = serving.Server(hbm_budget=8 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 22, probe=48)
+    srv.serve(reqs)
+    assert srv.max_concurrent >= 2
+    for r in reqs:
+        assert r.error is None, r.error
+        assert len(r.samples) == 48
+        sa = normalized_arrays(workloads.config2(natom=r.natom, seed=r.seed))
+        qs = [q for q, _ in r.samples]
+        got = np.concatenate([v for _, v in r.samples])
+        ref, offs, sizes = oracle.eri(sa, qs, deriv=1 if r.kind == "deriv" else 0)
+        if r.kind == "deriv":
+            ex, _, _ = oracle.md_eri_deriv1_analytic(sa, qs)
+            ok, st = parity_report_deriv(got, ref, offs, sizes, ex)
+        else:
+            ex, _, _ = oracle.md_eri(sa, qs, reference_boys=True)
+            ok, st = parity_report(got, ref, offs, sizes, ex)
+        print(r.rid, r.natom, r.kind, st)
+        assert ok, (r.rid, st)
 
 
 def test_admission_respects_budget(gpu):
