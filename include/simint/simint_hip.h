@@ -54,6 +54,25 @@ long long simint_hip_compute_eri_deriv1_list(simint_hip_pairs const * P, simint_
                                              long long const * d_qout, double screen_tol, double * d_work,
                                              double * d_out);
 
+/* No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same as
+ * simint_hip_compute_eri_deriv1_list, except that the derivative scratch is a
+ * fixed array inside the library (SIMINT_HIP_NOALLOC_SCRATCH_DOUBLES doubles,
+ * 64 MiB, held by the device until the process ends) instead of d_work, so
+ * the call allocates nothing.  simint_init must have run first (it looks up
+ * the array's device address); a list whose scratch,
+ * simint_hip_deriv1_worksize(P, Q, nquartet) doubles, exceeds the array is
+ * refused with SIMINT_ERROR_ARGUMENT before anything is queued.  Every call
+ * on the device shares the one array and leaves the last call's values in
+ * it (nothing needs to start at zero): calls that overlap on the device,
+ * from other threads or on other streams, race and must be ordered by the
+ * caller. */
+#define SIMINT_HIP_NOALLOC_SCRATCH_DOUBLES (8ull << 20)
+long long simint_hip_compute_eri_deriv1_list_noalloc(simint_hip_pairs const * P, simint_hip_pairs const * Q,
+                                                     long long nquartet, int const * d_qbra, int const * d_qket,
+                                                     long long const * d_qout, double screen_tol, double * d_out);
+
 /* First derivatives over ket ranges (as _range, FP64): quartet (b0+i, k),
  * k in [ket_lo[i], ket_hi[i]), written densely in (bra, ket) order -- quartet
  * number sum_{j<i}(ket_hi[j]-ket_lo[j]) + (k - ket_lo[i]) at d_out + number *

@@ -49,9 +49,17 @@ struct DeviceState {
     std::atomic<bool> ready{false};   // set once the device is fully set up
     int id = 0;
     double *boys = nullptr;
+    double *scratch = nullptr;        // device address of the fixed scratch area
     const D1Entry *d1tab[D1_N][D1_N][D1_N][D1_N] = {};
 };
 DeviceState g_dev;
+
+// Fixed scratch area of simint_hip_compute_eri_deriv1_list_noalloc: the
+// (a+1) / (a-1) class blocks of a derivative list live here instead of in a
+// caller-supplied work buffer.  The device keeps it for the life of the
+// process; every call on the device shares it.
+constexpr size_t kNoAllocScratchDoubles = SIMINT_HIP_NOALLOC_SCRATCH_DOUBLES;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 
 thread_local std::string t_err;
 
@@ -141,6 +149,12 @@ void device_init()
                 for (int dd = 0; dd < D1_N; dd++) g_dev.d1tab[a][b][c][dd] = dt + start[k++];
     g_dev.id = id;
     g_dev.boys = d;
+    // the host name `scratch` only stands for the device array: look up its
+    // device address once, with this device current, and use that
+    void *sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) { set_err("hipGetSymbolAddress(scratch)", e); return; }
+    g_dev.scratch = static_cast<double *>(sp);
     g_dev.ok = true;
     g_dev.ready.store(true, std::memory_order_release);
 }
@@ -1166,6 +1180,34 @@ long long simint_hip_compute_eri_deriv1_list(const simint_hip_pairs *P, const si
         return SIMINT_ERROR_ARGUMENT;
     if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
     return deriv1_list(P, Q, nquartet, d_qbra, d_qket, d_qout, screen_tol, d_work, d_out);
+}
+
+long long simint_hip_compute_eri_deriv1_list_noalloc(const simint_hip_pairs *P, const simint_hip_pairs *Q,
+                                                     long long nquartet, const int *d_qbra, const int *d_qket,
+                                                     const long long *d_qout, double screen_tol, double *d_out)
+{
+    if (!P || !Q || nquartet < 0 || (nquartet > 0 && (!d_qbra || !d_qket || !d_out))) return SIMINT_ERROR_ARGUMENT;
+    // no set-up here (it allocates): simint_init has looked up the array
+    if (!g_dev.ready.load(std::memory_order_acquire)) {
+        t_err = "call simint_init before simint_hip_compute_eri_deriv1_list_noalloc";
+        return SIMINT_ERROR_ARGUMENT;
+    }
+    if (hipSetDevice(g_dev.id) != hipSuccess) return SIMINT_ERROR_NO_DEVICE;
+    const int am[4] = {P->am1, P->am2, Q->am1, Q->am2};
+    if (!deriv1_built(am[0], am[1], am[2], am[3])) {
+        t_err = "first derivatives not built for this class";
+        return SIMINT_ERROR_UNSUPPORTED;
+    }
+    const unsigned long long need = (unsigned long long)deriv1_plan(am).scratch * (unsigned long long)nquartet;
+    if (need > kNoAllocScratchDoubles) {
+        char buf[256];
+        snprintf(buf, sizeof(buf),
+                 "derivative list needs %llu doubles of scratch; the fixed area holds %llu (split the list)",
+                 need, (unsigned long long)kNoAllocScratchDoubles);
+        t_err = buf;
+        return SIMINT_ERROR_ARGUMENT;
+    }
+    return deriv1_list(P, Q, nquartet, d_qbra, d_qket, d_qout, screen_tol, g_dev.scratch, d_out);
 }
 
 long long simint_hip_compute_eri_deriv1_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0,

@@ -64,6 +64,7 @@ EXPORTS = [
     "simint_compute_overlap", "simint_compute_ke", "simint_compute_potential",
     "simint_compute_osoei_overlap", "simint_compute_osoei_ke", "simint_compute_osoei_potential",
     "simint_hip_shells_create", "simint_hip_shells_free", "simint_hip_compute_osoei",
+    "simint_hip_compute_eri_deriv1_list_noalloc",
 ]
 
 _LIB = None
