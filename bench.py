@@ -553,6 +553,8 @@ def main():
         print(json.dumps({"copy_bytes": 8 * n}))
         return
     sys.path.insert(0, PKG)
+    # the CPU oracle (error check and cpu_baseline only): oracle/oracle.py
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import simint
     if args.tuning:
         simint.apply_tuning(args.tuning)

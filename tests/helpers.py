@@ -34,12 +34,26 @@ def per_quartet_errors(x, ref, offs, sizes):
 
 ATOL = 1e-14     # skel/test/test_eri.cpp:227 flags |diff| > 1e-14 (with rel > 1e-8)
 NORMWISE = 1e-14
+# magnitudes (largest |integral| of a quartet) at which the normwise error
+# statistics are reported
+FLOORS = (1e-12, 1e-10, 1e-8, 1e-6, 1e-4, 1e-3, 1e-2, 1e-1)
 
 
-def parity_report(got, ref, offs, sizes, exact=None):
+REL_FLOOR = 0.1     # quartets at least this large: normwise 1e-14 against the oracle
+
+
+def parity_report(got, ref, offs, sizes, exact=None, floor=REL_FLOOR):
     """Compare a result with the oracle per shell quartet.
 
     Pass criteria:
+      * per quartet whose largest |integral| is at least `floor`, the
+        normwise error max|got - ref| / max|ref| <= 1e-14 (the relative
+        bar; None disables it).  Below the floor the error is absolute: the
+        reference algorithm's own FP64 error against the exact values grows
+        there, from cancellation in the HRR and the contraction of quartets
+        whose integrals are small against their intermediates (up to 3e-6
+        normwise on the batched f-class stream, 7e-13 on the 12-atom system
+        of the 64-atom recipe), so two FP64 evaluations differ by that much;
       * elementwise |got - ref| <= 1e-14 * max(1, |ref|), or, when `exact`
         (long-double McMurchie-Davidson values) is given, <= 4 * that
         quartet's max|ref - exact| if larger: where the reference algorithm
@@ -78,12 +92,21 @@ def parity_report(got, ref, offs, sizes, exact=None):
         batch_ok = bool(worst_ok and med_ok)
         extra = {"gpu_worst_vs_exact": float(eg.max()), "oracle_worst_vs_exact": float(eo.max()),
                  "gpu_median_vs_exact": float(np.median(eg)), "oracle_median_vs_exact": float(np.median(eo))}
+    qmax = np.array([np.abs(ref[o:o + n]).max() if n else 0.0 for o, n in zip(offs, sizes)])
+    above = {"%g" % f: float(nw[qmax >= f].max()) if np.any(qmax >= f) else 0.0 for f in FLOORS}
     stats = {"max_abs": float(d.max()) if d.size else 0.0,
              "max_normwise": float(nw.max()) if nw.size else 0.0,
+             "max_normwise_above_floor": above,
              "frac_normwise_gt_1e-14": float((nw > NORMWISE).mean()) if nw.size else 0.0,
              "median_normwise": float(np.median(nw)) if nw.size else 0.0}
+    rel_ok = True
+    if floor is not None and len(offs):
+        big = qmax >= floor
+        rel_ok = bool(not np.any(big) or nw[big].max() <= NORMWISE)
+        stats["floor"] = floor
+        stats["quartets_above_floor"] = int(big.sum())
     stats.update(extra)
-    return elem_ok and batch_ok, stats
+    return elem_ok and batch_ok and rel_ok, stats
 
 
 DERIV_ABS = 1e-10   # the absolute ERI error bound the engine is held to

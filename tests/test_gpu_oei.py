@@ -2,9 +2,17 @@
 _potential and the batched simint_hip_compute_osoei) against the oracle's
 restatement of skel/simint/osoei and its long-double Hermite expansions.
 
-Bar: per shell pair, max|gpu - oracle| <= 1e-14 max|oracle| (normwise); the
-nuclear attraction additionally within the Boys scheme's own error of the
-exact long-double values."""
+Bar (helpers.parity_report, as for the ERIs), with the long-double Hermite
+values as "exact" (for the nuclear attraction fed simint's Boys values):
+elementwise |gpu - oracle| <= 1e-14 max(1, |oracle|), or 4x that block's
+own oracle error against the exact values where larger; and over the batch
+the GPU at least as accurate as the oracle (worst and median normwise error
+against the exact values within 4x the oracle's).  The kinetic-energy
+tables cancel (T_00 = S_00 (a - 2a^2 (PA^2 + 1/2p))), so two FP64
+evaluations that round differently (fused multiply-adds on the GPU) differ
+by a few 1e-14 normwise on some blocks (the GPU being the closer of the two
+to the exact values there), so the kinetic energy is held to the exact-value
+criteria without the relative bar against the oracle."""
 import ctypes
 
 import numpy as np
@@ -13,7 +21,7 @@ import torch
 
 import oracle
 import simint
-from helpers import normalized_arrays
+from helpers import REL_FLOOR, normalized_arrays, normwise_rel, parity_report
 
 pytestmark = pytest.mark.gpu
 
@@ -32,8 +40,9 @@ def _shells(seed, lmax=3, per_l=2, depth=(1, 5), lo=0.05, hi=80.0, box=2.5):
 
 
 def _worst(a, b, off, n):
-    return max(np.abs(a[o:o + m] - b[o:o + m]).max() / max(np.abs(b[o:o + m]).max(), 1e-300)
-               for o, m in zip(off, n))
+    """Worst normwise error per block (blocks whose values vanish below 1e-12,
+    by cancellation between far-apart shells, by absolute difference)."""
+    return max(normwise_rel(a[o:o + m], b[o:o + m]) for o, m in zip(off, n))
 
 
 CHARGES = (np.array([1.0, 6.0, 8.0, 92.0]), np.random.default_rng(7).uniform(-2.5, 2.5, (4, 3)))
@@ -55,10 +64,13 @@ def test_dropin_oei_vs_oracle(gpu, kind):
         rc, v = simint.compute_oei(kind, basis, i, j, CHARGES)
         assert rc == 1, (rc, simint.last_error())
         got[o:o + m] = v
-    w = _worst(got, ref, off, n)
-    print(kind, "worst normwise vs oracle", w)
-    assert w <= TOL, w
+    ex, _, _ = oracle.md_oei(sa, pairs, k, Z if k == 2 else None, C if k == 2 else None, reference_boys=True)
+    # kinetic energy: no relative bar against the oracle (see the module text)
+    ok, st = parity_report(got, ref, off, n, ex, floor=None if k == 1 else REL_FLOOR)
+    print(kind, st)
+    assert ok, st
     if k == 2:
+        # and against exact Boys values: within the Boys scheme's own error
         ex, _, _ = oracle.md_oei(sa, pairs, 2, Z, C)
         eo = _worst(ref, ex, off, n)
         eg = _worst(got, ex, off, n)
@@ -95,9 +107,11 @@ def test_batched_oei_g_h_shells(gpu):
             simint.synchronize()
             h = out.cpu().numpy()
             got = np.concatenate([h[offs[q]:offs[q] + n[q]] for q in range(len(pairs))])
-            w = _worst(got, ref, off, n)
-            print(kind, "g/h batch worst normwise vs oracle", w)
-            assert w <= TOL, (kind, w)
+            ex, _, _ = oracle.md_oei(sa, pairs, k, Z if k == 2 else None, C if k == 2 else None,
+                                     reference_boys=True)
+            ok, st = parity_report(got, ref, off, n, ex, floor=None if k == 1 else REL_FLOOR)
+            print(kind, "g/h batch", st)
+            assert ok, (kind, st)
     finally:
         ds.free()
         basis.free()

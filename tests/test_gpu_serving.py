@@ -63,7 +63,7 @@ def test_served_results_vs_oracle(gpu):
             ok, st = parity_report_deriv(got, ref, offs, sizes, ex)
         else:
             ex, _, _ = oracle.md_eri(sa, qs, reference_boys=True)
-            ok, st = parity_report(got, ref, offs, sizes, ex)
+            ok, st = parity_report(got, ref, offs, sizes, ex, floor=1e-2)
         print(r.rid, r.natom, r.kind, st)
         assert ok, (r.rid, st)
 
