@@ -94,6 +94,8 @@ def kernel_symbol(info, cls, precision=0):
         return ("eri_lpqm_%s" if precision == 1 else "eri_lpq_%s") % name
     if fam in (3, 4):
         return "eri_" + info[name]["split_kernels"][fam - 3]
+    if fam in (5, 6):
+        return "eri_" + info[name]["split_kernels_f"][fam - 5]
     return "eri_wpq_%s" % name
 
 

@@ -21,6 +21,8 @@ typedef struct simint_hip_pairs simint_hip_pairs;
 int  simint_hip_pairs_create(struct simint_multi_shellpair const * P, simint_hip_pairs ** out);
 void simint_hip_pairs_free(simint_hip_pairs * p);
 int  simint_hip_pairs_count(simint_hip_pairs const * p);
+/* device memory (bytes) the pair set holds */
+size_t simint_hip_pairs_bytes(simint_hip_pairs const * p);
 
 /* (ab|cd) for bra pairs b = b0 .. b0+nbra-1 of P and, for each, ket pairs
  * k = 0 .. nket[b-b0]-1 of Q: the per-bra prefix a Schwarz screen over

@@ -1,8 +1,9 @@
 """Per-class kernel-family autotuner (runs on the GPU).
 
 For every class, times each kernel family built for it -- lane-per-quartet
-(registers), wave-per-quartet (LDS-staged, batched) and the split
-lane-per-quartet kernels with two group counts -- on the screened quartets
+(registers), wave-per-quartet (LDS-staged, batched), the split
+lane-per-quartet kernels with two group counts and, for bra-heavy d
+classes, their ket-top-group variants -- on the screened quartets
 of a synthetic system built with the 64-atom benchmark recipe at a smaller
 size, and writes the fastest family per class to tuning.json (read by the
 build to set each class's default):
@@ -48,7 +49,7 @@ def main():
         name = quartets.simint_class_name(cls)
         prev = L.simint_hip_class_kernel(*cls)
         res = {}
-        for fam, fname in ((1, "lpq"), (2, "wpq"), (3, "lpqs_a"), (4, "lpqs_b")):
+        for fam, fname in ((1, "lpq"), (2, "wpq"), (3, "lpqs_a"), (4, "lpqs_b"), (5, "lpqsf_a"), (6, "lpqsf_b")):
             if L.simint_hip_set_class_kernel(*cls, fam) < 0:
                 continue
             best = None

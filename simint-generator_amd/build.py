@@ -117,7 +117,10 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
     from emitter.hip_lpq import class_name
     want = set("eri_lpq_%s.hip" % class_name(a) for a in lpq) | set("eri_wpq_%s.hip" % class_name(a) for a in wpq)
     from emitter.hip_lpqs import kernel_name
-    want |= set("eri_%s.hip" % kernel_name(a, G) for a, Gs in lpqs.items() for G in Gs)
+    # split kernels: keys (a,b,c,d) for the default kind, ("f", a,b,c,d) for
+    # the ket-top variants of bra-heavy classes
+    want |= set("eri_%s.hip" % (kernel_name(a[1:], G, "f") if a[0] == "f" else kernel_name(a, G))
+                for a, Gs in lpqs.items() for G in Gs)
     srcs = [s for s in srcs if os.path.basename(s) in want]
     rt = [os.path.join(CSRC, "runtime.hip"), os.path.join(CSRC, "osoei.hip"), os.path.join(CSRC, "shell.cpp"),
           os.path.join(CSRC, "shell_screen.cpp")]

@@ -42,6 +42,8 @@ struct D1Entry {
     int ip, im, ex, pad;
 };
 constexpr int D1_N = SIMINT_OSTEI_DERIV1_MAXAM + 1;
+// the (a+1) classes of a derivative reach one AM above the derivative maximum
+constexpr int kTermMaxAM = SIMINT_OSTEI_DERIV1_MAXAM + 1;
 
 struct DeviceState {
     std::once_flag once;
@@ -621,8 +623,8 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
     }
     size_t scratch = 0;
     for (auto &t : terms) {
-        if (t.am[0] > SIMINT_OSTEI_MAXAM || t.am[1] > SIMINT_OSTEI_MAXAM || t.am[2] > SIMINT_OSTEI_MAXAM ||
-            t.am[3] > SIMINT_OSTEI_MAXAM || !(t.fn = simint_dispatch(t.am[0], t.am[1], t.am[2], t.am[3]))) {
+        if (t.am[0] > kTermMaxAM || t.am[1] > kTermMaxAM || t.am[2] > kTermMaxAM || t.am[3] > kTermMaxAM ||
+            !(t.fn = simint_dispatch(t.am[0], t.am[1], t.am[2], t.am[3]))) {
             t_err = "class not built";
             return SIMINT_ERROR_UNSUPPORTED;
         }
@@ -765,6 +767,7 @@ struct simint_hip_pairs {
     PairsLinear lin;
     PairsTiled til;
     int ntile;
+    size_t bytes = 0;                            // device memory held
     const double *wa = nullptr, *wb = nullptr;   // linear prefac * alpha2, prefac * beta2
     const double *twa = nullptr, *twb = nullptr; // the same in the tiled view
 };
@@ -793,6 +796,7 @@ int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **
     p->screen_max = P->screen_max;
     p->has_screen = P->screen != nullptr;
     p->dev = d;
+    p->bytes = im.buf.size();
     p->lin = linear_ptrs(static_cast<char *>(d), lo);
     p->til = tiled_ptrs(static_cast<char *>(d), to);
     p->ntile = to.ntile;
@@ -812,6 +816,8 @@ void simint_hip_pairs_free(simint_hip_pairs *p)
 }
 
 int simint_hip_pairs_count(const simint_hip_pairs *p) { return p ? p->n : 0; }
+
+size_t simint_hip_pairs_bytes(const simint_hip_pairs *p) { return p ? p->bytes : 0; }
 
 }  // extern "C"
 
@@ -1130,8 +1136,7 @@ long long deriv1_list(const simint_hip_pairs *P, const simint_hip_pairs *Q, long
     for (int k = 0; k < dp.nt; k++) {
         const DerivPlan::T &t = dp.t[k];
         fns[k] = nullptr;
-        if (t.am[0] <= SIMINT_OSTEI_MAXAM && t.am[1] <= SIMINT_OSTEI_MAXAM && t.am[2] <= SIMINT_OSTEI_MAXAM &&
-            t.am[3] <= SIMINT_OSTEI_MAXAM)
+        if (t.am[0] <= kTermMaxAM && t.am[1] <= kTermMaxAM && t.am[2] <= kTermMaxAM && t.am[3] <= kTermMaxAM)
             fns[k] = simint_dispatch_list(t.am[0], t.am[1], t.am[2], t.am[3]);
         if (!fns[k]) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
     }
@@ -1226,8 +1231,7 @@ long long simint_hip_compute_eri_deriv1_range(const simint_hip_pairs *P, const s
     for (int k = 0; k < dp.nt; k++) {
         const DerivPlan::T &t = dp.t[k];
         terms[k] = RangeTerm{{t.am[0], t.am[1], t.am[2], t.am[3]}, nullptr, t.weight ? t.centre : -1};
-        if (t.am[0] <= SIMINT_OSTEI_MAXAM && t.am[1] <= SIMINT_OSTEI_MAXAM && t.am[2] <= SIMINT_OSTEI_MAXAM &&
-            t.am[3] <= SIMINT_OSTEI_MAXAM)
+        if (t.am[0] <= kTermMaxAM && t.am[1] <= kTermMaxAM && t.am[2] <= kTermMaxAM && t.am[3] <= kTermMaxAM)
             terms[k].fn = simint_dispatch(t.am[0], t.am[1], t.am[2], t.am[3]);
         if (!terms[k].fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
     }

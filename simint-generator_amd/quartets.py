@@ -309,6 +309,10 @@ class ScreenedSet:
             out.append((ln.chunk, int(ln.off[bi]) + j * nc, nc, (bp[0], bp[1], kp[0], kp[1])))
         return out
 
+    def device_bytes(self):
+        """Device memory the set holds (its shell-pair groups)."""
+        return sum(g.dev.bytes for g in self.groups)
+
     def free(self):
         for g in self.groups:
             g.dev.free()

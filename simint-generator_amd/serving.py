@@ -1,20 +1,30 @@
 """Continuous-batching ERI service on one GPU.
 
 Requests arrive as a Poisson stream.  Each asks for the Schwarz-screened
-ERIs (or first derivatives) of one synthetic molecule built with the
-64-atom-system recipe (its own atom count and seed).  The server
+ERIs (or first derivatives) of one molecule: a synthetic one built with the
+64-atom-system recipe (its own atom count and seed), or explicit shells.
+The server
 
-  * admits a request when a worker slot is free and its estimated device
-    footprint (shell-pair data, an output chunk buffer, derivative scratch)
-    fits the remaining HBM budget -- requests that arrive while others run
-    are admitted as slots and memory free up, not after the queue drains;
-  * runs admitted requests concurrently, each on its own worker thread and
-    so on its own HIP stream (the library keeps one stream per host
-    thread), so their kernels overlap on the device;
-  * keeps each request's shell pairs resident in HBM while it runs, streams
-    its integrals chunk by chunk through one output buffer, and reduces every
-    chunk on the device into a digest (sum of values and sum of squares in
-    FP64), the stand-in for a consumer such as a Fock build.
+  * rejects a malformed request (a non-finite or non-positive exponent, a
+    non-finite coefficient or centre, an angular momentum above d, an empty
+    shell) before it touches the device, without disturbing the others;
+  * admits a request when its estimated device footprint fits the
+    remaining HBM budget, then charges what its set-up actually allocated
+    (the shell-pair data it keeps resident) -- requests that arrive while
+    others run are admitted as memory frees, not after the queue drains;
+  * cuts every admitted request into tiles (one output chunk of its
+    screened quartet set each) and lets its worker threads -- one HIP
+    stream each, so their kernels overlap on the device -- take tiles across
+    requests, the request with the least work left first: a large request
+    no longer holds a slot for its whole life, and small ones that arrive
+    behind it finish without waiting for it;
+  * reduces every tile on the device into that request's per-tile digest
+    (sum of values and sum of squares in FP64, the stand-in for a consumer
+    such as a Fock build) and sums the per-tile digests in tile order at the
+    end, so a request's result does not depend on how its tiles were
+    scheduled -- it is the same alone and in the mix;
+  * stops scheduling a request's tiles once it has run past its time limit
+    and reports it as timed out.
 
 Latency is arrival to completion; throughput is shell quartets (derivative
 quartets counted once) per second of wall time.
@@ -23,6 +33,7 @@ quartets counted once) per second of wall time.
 """
 import argparse
 import json
+import math
 import os
 import queue
 import sys
@@ -39,15 +50,29 @@ import quartets  # noqa: E402
 import simint  # noqa: E402
 import workloads  # noqa: E402
 
+MAXAM = 2   # shells the screened sets take (the recipe's s, p, d)
+
 
 class Request:
-    def __init__(self, rid, natom, seed, kind, arrival):
+    def __init__(self, rid, natom, seed, kind, arrival, shells=None, timeout=None):
         self.rid, self.natom, self.seed, self.kind, self.arrival = rid, natom, seed, kind, arrival
+        self.shells = shells       # explicit shells, or None: the recipe for (natom, seed)
+        self.timeout = timeout     # seconds after admission, or None
         self.t_admit = self.t_done = None
         self.quartets = 0
         self.digest = None
         self.samples = None        # [(quartet, values)] when the server probes results
         self.error = None
+        # scheduling state
+        self.ss = None
+        self.sums = None
+        self.next = 0
+        self.done_tiles = 0
+        self.inflight = 0
+        self.charged = 0.0
+        self.probes = {}
+        self.kept = []
+        self.left = 0.0
 
     def to_dict(self):
         return {"id": self.rid, "natom": self.natom, "seed": self.seed, "kind": self.kind,
@@ -70,20 +95,36 @@ def request_stream(n, rate, seed=0, natom_range=(10, 200), deriv_frac=0.3):
     return out
 
 
-def footprint(natom, kind, chunk_doubles):
-    """Device bytes a request may hold: pair data (linear and tiled views,
-    ~20 doubles per primitive pair, 6.25 primitive pairs per pair on
-    average), the output chunk and, for derivatives, the class scratch."""
-    nsh = 6 * natom
-    npair = nsh * (nsh + 1) // 2
-    pairs = npair * 6.25 * 20 * 8 * 2
-    out = 8.0 * chunk_doubles
-    work = 8.0 * chunk_doubles * 4 if kind == "deriv" else 0.0
-    return pairs + out + work
+def validate(shells):
+    """None if the shells can be served, else the reason they cannot."""
+    if not shells:
+        return "no shells"
+    for k, (am, xyz, al, co) in enumerate(shells):
+        if not isinstance(am, (int, np.integer)) or am < 0 or am > MAXAM:
+            return "shell %d: angular momentum %r (0..%d served)" % (k, am, MAXAM)
+        if len(al) == 0 or len(al) != len(co):
+            return "shell %d: %d exponents, %d coefficients" % (k, len(al), len(co))
+        if not all(math.isfinite(float(x)) for x in xyz):
+            return "shell %d: centre not finite" % k
+        if not all(math.isfinite(float(a)) and float(a) > 0.0 for a in al):
+            return "shell %d: exponent not finite and positive" % k
+        if not all(math.isfinite(float(c)) for c in co):
+            return "shell %d: coefficient not finite" % k
+    return None
+
+
+def footprint(nshell, kind, chunk_doubles):
+    """Estimated device bytes a request will hold before its set-up has run:
+    its shell pairs (linear and tiled views, ~20 doubles per primitive pair,
+    6.25 primitive pairs per pair on average).  The output chunk buffers and
+    the derivative scratch belong to the workers, not to requests."""
+    npair = nshell * (nshell + 1) // 2
+    return npair * 6.25 * 20 * 8 * 2
 
 
 class Server:
-    def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False, probe=0):
+    def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False, probe=0,
+                 timeout=None):
         self.verbose = verbose
         self.probe = probe         # quartets per request whose values are kept (result checks)
         self.budget = float(hbm_budget)
@@ -91,94 +132,182 @@ class Server:
         self.slots = slots
         self.tol = tol
         self.chunk = chunk_doubles
+        self.timeout = timeout     # default per-request time limit (s after admission)
         self.cv = threading.Condition()
         self.waiting = []          # arrived, not admitted (arrival order)
+        self.active = []           # admitted, tiles left or in flight
+        self.setting_up = 0
         self.arrivals_done = False
         self.t0 = None
         self.max_concurrent = 0
-        self.running = 0
+        self.tiles_run = 0
 
     def now(self):
         return time.perf_counter() - self.t0
 
-    # ------------------------------------------------------------ admission
-    def _admissible(self):
-        """First waiting request (arrival order) that fits the free budget."""
+    def _shells(self, r):
+        return r.shells if r.shells is not None else workloads.config2(natom=r.natom, seed=r.seed)
+
+    @staticmethod
+    def _nshell(r):
+        # the recipe has six shells per atom
+        return len(r.shells) if r.shells is not None else 6 * r.natom
+
+    # ------------------------------------------------------------ scheduling
+    def _next_task(self):
+        """Under the lock: ('tile', r, c), ('setup', r) or None.  Requests past
+        their time limit stop getting tiles and end once none is in flight;
+        a request larger than the whole budget is refused."""
+        now = self.now()
+        live = []
+        for r in list(self.active):
+            lim = r.timeout if r.timeout is not None else self.timeout
+            if r.error is None and lim is not None and now - r.t_admit > lim:
+                r.error = "timeout after %.1f s" % (now - r.t_admit)
+            if r.error is not None and r.inflight == 0:
+                self._finish(r)
+                self._complete(r, self.done)
+            elif r.error is None and r.next < r.ss.nchunk:
+                live.append(r)
+        for r in list(self.waiting):
+            if footprint(self._nshell(r), r.kind, self.chunk) > self.budget:
+                self.waiting.remove(r)
+                r.error = "rejected: footprint above the HBM budget"
+                r.t_admit = r.t_done = now
+                self._complete(r, self.done)
+        if live:
+            r = min(live, key=lambda x: (x.left, x.arrival))
+            c = r.next
+            r.next += 1
+            r.inflight += 1
+            return ("tile", r, c)
         for r in self.waiting:
-            if footprint(r.natom, r.kind, self.chunk) <= self.free:
-                return r
+            if footprint(self._nshell(r), r.kind, self.chunk) <= self.free:
+                self.waiting.remove(r)
+                est = footprint(self._nshell(r), r.kind, self.chunk)
+                self.free -= est
+                r.charged = est
+                r.t_admit = now
+                self.setting_up += 1
+                return ("setup", r)
         return None
+
+    def _finish(self, r):
+        """Under the lock: the request has no tile left in flight."""
+        self.active.remove(r)
+        self.free += r.charged
+        r.t_done = self.now()
+
+    def _setup(self, r):
+        import torch
+        shells = self._shells(r)
+        bad = validate(shells)
+        if bad is not None:
+            r.error = "rejected: " + bad
+            return
+        deriv = 1 if r.kind == "deriv" else 0
+        ss = quartets.ScreenedSet(shells, tol=self.tol, out_doubles=self.chunk, deriv=deriv)
+        r.ss = ss
+        r.quartets = ss.total_quartets
+        r.left = float(ss.total_bytes)
+        r.sums = torch.zeros((max(ss.nchunk, 1), 2), dtype=torch.float64, device="cuda")
+        r.work_need = ss.deriv_worksize() if deriv else 0
+        torch.cuda.synchronize()
+        if self.probe:
+            for c, off, nc, q in ss.located_sample(self.probe, seed=r.seed):
+                r.probes.setdefault(c, []).append((off, nc, q))
+
+    def _tile(self, r, c, out, stream, work):
+        """Run output chunk c of request r on this worker's stream and reduce
+        it into r's digest slot c."""
+        import torch
+        ss = r.ss
+        n = ss.chunk_doubles(c)
+        wp = None
+        if ss.deriv:
+            need = r.work_need
+            if work[0] is None or work[0].numel() < need:
+                work[0] = torch.empty(need, dtype=torch.float64, device="cuda")
+            wp = work[0].data_ptr()
+        with torch.cuda.stream(stream):
+            ss.run_pass(out.data_ptr(), chunk=c, work_ptr=wp)
+            x = out[:n]
+            r.sums[c, 0] = x.sum()
+            r.sums[c, 1] = torch.dot(x, x)
+            kept = [(q, out[off:off + nc].clone()) for off, nc, q in r.probes.get(c, [])]
+            stream.synchronize()
+        return [(q, v.cpu().numpy()) for q, v in kept], sum(8.0 * ln.nout for ln in ss.chunk_launches(c))
 
     def _worker(self, done):
         import torch
         stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
         out = torch.empty(self.chunk, dtype=torch.float64, device="cuda")
+        work = [None]
         while True:
             with self.cv:
                 while True:
-                    r = self._admissible()
-                    if r is not None:
+                    task = self._next_task()
+                    if task is not None:
                         break
-                    if self.arrivals_done and not self.waiting:
+                    if self.arrivals_done and not self.waiting and not self.active and not self.setting_up:
                         return
-                    self.cv.wait()
-                self.waiting.remove(r)
-                fp = footprint(r.natom, r.kind, self.chunk)
-                self.free -= fp
-                self.running += 1
-                self.max_concurrent = max(self.max_concurrent, self.running)
-                r.t_admit = self.now()
+                    self.cv.wait(timeout=0.5)
+            if task[0] == "setup":
+                r = task[1]
+                try:
+                    self._setup(r)
+                except Exception as e:      # reported per request, the service goes on
+                    r.error = repr(e)
+                with self.cv:
+                    self.setting_up -= 1
+                    if r.ss is not None:
+                        actual = float(r.ss.device_bytes()) + r.sums.numel() * 8.0
+                        self.free += r.charged - actual
+                        r.charged = actual
+                    if r.error is None and r.ss is not None and r.ss.nchunk > 0:
+                        self.active.append(r)
+                        self.max_concurrent = max(self.max_concurrent, len(self.active))
+                    else:
+                        self.active.append(r)
+                        self._finish(r)
+                        self._complete(r, done)
+                    self.cv.notify_all()
+                continue
+            _, r, c = task
+            kept, nbytes = [], 0.0
+            err = None
             try:
-                self._process(r, out, stream)
-            except Exception as e:      # reported per request, the service goes on
-                r.error = repr(e)
-            r.t_done = self.now()
+                kept, nbytes = self._tile(r, c, out, stream, work)
+            except Exception as e:
+                err = repr(e)
             with self.cv:
-                self.free += fp
-                self.running -= 1
+                self.tiles_run += 1
+                r.inflight -= 1
+                r.done_tiles += 1
+                r.left -= nbytes
+                r.kept.extend(kept)
+                if err is not None and r.error is None:
+                    r.error = err
+                if r.inflight == 0 and (r.done_tiles == r.ss.nchunk or r.error is not None):
+                    self._finish(r)
+                    self._complete(r, done)
                 self.cv.notify_all()
-            if self.verbose:
-                print("[serve %.1fs] request %d (%d atoms, %s) done: %.2f s after arrival, %d quartets%s"
-                      % (r.t_done, r.rid, r.natom, r.kind, r.t_done - r.arrival, r.quartets,
-                         "" if r.error is None else ", error " + r.error), file=sys.stderr, flush=True)
-            done.put(r)
 
-    def _process(self, r, out, stream):
-        import torch
-        shells = workloads.config2(natom=r.natom, seed=r.seed)
-        deriv = 1 if r.kind == "deriv" else 0
-        ss = quartets.ScreenedSet(shells, tol=self.tol, out_doubles=self.chunk, deriv=deriv)
-        work = None
-        if deriv:
-            work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
-        t_note = time.perf_counter()
-        probes = {}
-        if self.probe:
-            for c, off, nc, q in ss.located_sample(self.probe, seed=r.seed):
-                probes.setdefault(c, []).append((off, nc, q))
-        kept = []
-        with torch.cuda.stream(stream):
-            # zeroed on the library stream, so the fill is ordered before the
-            # accumulations that stream runs
-            s = torch.zeros(2, dtype=torch.float64, device="cuda")
-            for c in range(ss.nchunk):
-                if self.verbose and time.perf_counter() - t_note > 30.0:
-                    print("[serve %.1fs] request %d: chunk %d of %d" % (self.now(), r.rid, c, ss.nchunk),
-                          file=sys.stderr, flush=True)
-                    t_note = time.perf_counter()
-                n = ss.chunk_doubles(c)
-                ss.run_pass(out.data_ptr(), chunk=c, work_ptr=None if work is None else work.data_ptr())
-                x = out[:n]
-                s[0] += x.sum()
-                s[1] += torch.dot(x, x)
-                for off, nc, q in probes.get(c, []):
-                    kept.append((q, out[off:off + nc].clone()))
-            stream.synchronize()
-        r.quartets = ss.total_quartets
-        r.digest = [float(v) for v in s.cpu()]
-        if self.probe:
-            r.samples = [(q, v.cpu().numpy()) for q, v in kept]
-        ss.free()
+    def _complete(self, r, done):
+        """Under the lock: digest in tile order, release the request's data."""
+        if r.error is None and r.sums is not None:
+            r.digest = [float(v) for v in r.sums.sum(dim=0).cpu()]
+            if self.probe:
+                r.samples = sorted(r.kept, key=lambda qv: qv[0])
+        if r.ss is not None:
+            r.ss.free()
+            r.ss = None
+        r.sums = None
+        if self.verbose:
+            print("[serve %.1fs] request %d (%d atoms, %s) done: %.2f s after arrival, %d quartets%s"
+                  % (r.t_done, r.rid, r.natom, r.kind, r.t_done - r.arrival, r.quartets,
+                     "" if r.error is None else ", error " + r.error), file=sys.stderr, flush=True)
+        done.put(r)
 
     # ------------------------------------------------------------------ run
     def serve(self, requests):
@@ -188,6 +317,7 @@ class Server:
         simint.lib().simint_init()
         torch.cuda.init()
         done = queue.Queue()
+        self.done = done
         self.t0 = time.perf_counter()
         workers = [threading.Thread(target=self._worker, args=(done,), daemon=True) for _ in range(self.slots)]
         for w in workers:
@@ -229,12 +359,14 @@ def main():
     ap.add_argument("--min-atoms", type=int, default=10)
     ap.add_argument("--max-atoms", type=int, default=200)
     ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--timeout", type=float, default=None, help="per-request limit, seconds after admission")
     a = ap.parse_args()
     reqs = request_stream(a.requests, a.rate, a.seed, (a.min_atoms, a.max_atoms))
-    srv = Server(hbm_budget=a.budget_gib * (1 << 30), slots=a.slots, tol=a.tol, verbose=True)
+    srv = Server(hbm_budget=a.budget_gib * (1 << 30), slots=a.slots, tol=a.tol, verbose=True, timeout=a.timeout)
     srv.serve(reqs)
     res = summarize(reqs, srv.wall)
     res["max_concurrent"] = srv.max_concurrent
+    res["tiles"] = srv.tiles_run
     res["requests_detail"] = [r.to_dict() for r in reqs]
     print(json.dumps(res))
 

@@ -64,7 +64,7 @@ EXPORTS = [
     "simint_compute_overlap", "simint_compute_ke", "simint_compute_potential",
     "simint_compute_osoei_overlap", "simint_compute_osoei_ke", "simint_compute_osoei_potential",
     "simint_hip_shells_create", "simint_hip_shells_free", "simint_hip_compute_osoei",
-    "simint_hip_compute_eri_deriv1_list_noalloc",
+    "simint_hip_compute_eri_deriv1_list_noalloc", "simint_hip_pairs_bytes",
 ]
 
 _LIB = None
@@ -99,6 +99,8 @@ def lib():
         L.simint_hip_pairs_create.argtypes = [MP, ctypes.POINTER(c_void_p)]
         L.simint_hip_pairs_free.argtypes = [c_void_p]
         L.simint_hip_pairs_count.argtypes = [c_void_p]
+        L.simint_hip_pairs_bytes.argtypes = [c_void_p]
+        L.simint_hip_pairs_bytes.restype = c_size_t
         L.simint_hip_compute_eri_prefix.argtypes = [c_void_p, c_void_p, c_int, c_int, iptr, llptr, c_double, c_void_p]
         L.simint_hip_compute_eri_prefix.restype = ctypes.c_longlong
         L.simint_hip_compute_eri_diag.argtypes = [c_void_p, c_void_p]
@@ -256,6 +258,7 @@ class DevicePairs:
         self.h = h
         self.am = (pairs.mp.am1, pairs.mp.am2)
         self.n = pairs.mp.nshell12_clip
+        self.bytes = int(lib().simint_hip_pairs_bytes(h))
 
     def free(self):
         if self.h:
@@ -263,7 +266,7 @@ class DevicePairs:
             self.h = None
 
 
-FAMILIES = {"lpq": 1, "wpq": 2, "lpqs_a": 3, "lpqs_b": 4}
+FAMILIES = {"lpq": 1, "wpq": 2, "lpqs_a": 3, "lpqs_b": 4, "lpqsf_a": 5, "lpqsf_b": 6}
 
 
 def apply_tuning(path):

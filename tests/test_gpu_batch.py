@@ -163,19 +163,25 @@ def test_list_graph_replay(gpu):
     b.free()
 
 
-def test_deriv1_list(gpu):
-    """First derivatives through the list path against the oracle, every
-    class up to d (the (a+1) classes of d are f)."""
+@pytest.mark.parametrize("maxam", [2, 3])
+def test_deriv1_list(gpu, maxam):
+    """First derivatives through the list path against the oracle: every
+    class up to d (the (a+1) classes of d are f), and 48 classes with an f
+    shell (their (a+1) classes have a g shell)."""
     import torch
-    shells = [s for s in _pool(seed=6, per_am=6, maxk=3) if s[0] <= 2]
+    shells = [s for s in _pool(seed=6, per_am=6, maxk=3) if s[0] <= maxam]
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     rng = np.random.default_rng(7)
     byam = {}
     for n, s in enumerate(shells):
         byam.setdefault(s[0], []).append(n)
-    qs = np.array([[rng.choice(byam[a]) for a in am] for am in itertools.product(range(3), repeat=4)
-                   for _ in range(2)], dtype=np.int64)
+    if maxam == 2:
+        classes = [am for am in itertools.product(range(3), repeat=4) for _ in range(2)]
+    else:
+        fc = [am for am in itertools.product(range(4), repeat=4) if 3 in am]
+        classes = [fc[i] for i in rng.choice(len(fc), size=48, replace=False)]
+    qs = np.array([[rng.choice(byam[a]) for a in am] for am in classes], dtype=np.int64)
     b = batch.QuartetBatch(basis, qs, deriv=1)
     out = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
     b.run(out.data_ptr())

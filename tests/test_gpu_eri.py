@@ -183,14 +183,37 @@ def test_deriv1_screened_and_limits(gpu, screen):
     # same shell on both sides of a pair
     s0 = byam[1][0]
     _check(sa, basis, [s0], [s0], [s0], [s0], deriv=1)
-    # an f shell needs the (g..) classes for its derivative: above the
-    # compiled maximum AM of 3
+    # a g shell is above the compiled maximum AM (3), for integrals and
+    # derivatives alike
     d = byam[2][0]
-    b2 = simint.Basis([(3, shells[d][1], shells[d][2], shells[d][3])])
+    b2 = simint.Basis([(4, shells[d][1], shells[d][2], shells[d][3])])
     P = simint.Pairs(b2, [0], [0])
-    rc, _ = simint.compute_eri(P, P, 0.0, 1)
+    for der in (0, 1):
+        rc, _ = simint.compute_eri(P, P, 0.0, der)
+        assert rc == simint.ERROR_UNSUPPORTED, rc
     P.free()
-    assert rc == simint.ERROR_UNSUPPORTED, rc
+
+
+def test_deriv1_f_classes(gpu):
+    """First derivatives of every class with an f shell (SIMINT_OSTEI_DERIV1_MAXAM
+    equals the maximum AM, as in the reference, ostei_config.h.in:7): their
+    (a+1) terms are the classes with a g shell, built for derivatives only.
+    Two shells of each AM, every combination per class, against the oracle
+    and the exact long-double analytic derivatives."""
+    shells = _f_shells(seed=23)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    assert simint.lib().simint_eri_worksize(1, 3) > 0
+    worst = 0.0
+    for am in itertools.product(range(4), repeat=4):
+        if 3 not in am:
+            continue
+        st = _check(sa, basis, byam[am[0]], byam[am[1]], byam[am[2]], byam[am[3]], deriv=1)
+        worst = max(worst, st["max_abs_vs_oracle"])
+    print("f-class deriv1 worst |gpu - oracle|", worst)
 
 
 def test_lane_and_wave_kernels_agree(gpu):

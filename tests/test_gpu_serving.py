@@ -69,13 +69,58 @@ def test_served_results_vs_oracle(gpu):
 
 
 def test_admission_respects_budget(gpu):
-    """A budget that holds one request at a time serialises them."""
+    """A budget that holds one request at a time serialises them: a request
+    is admitted on its estimate and then charged what its set-up allocated."""
     reqs = [serving.Request(i, 6, 30 + i, "eri", 0.0) for i in range(3)]
-    fp = serving.footprint(6, "eri", 1 << 22)
-    srv = serving.Server(hbm_budget=fp * 1.5, slots=4, tol=1e-10, chunk_doubles=1 << 22)
+    est = serving.footprint(36, "eri", 1 << 22)
+    ss = quartets.ScreenedSet(workloads.config2(natom=6, seed=30), tol=1e-10, out_doubles=1 << 22)
+    act = ss.device_bytes()
+    ss.free()
+    assert act > 0
+    srv = serving.Server(hbm_budget=est + 0.5 * act, slots=4, tol=1e-10, chunk_doubles=1 << 22)
     srv.serve(reqs)
     assert srv.max_concurrent == 1
     assert all(r.error is None for r in reqs)
+
+
+def test_malformed_and_timed_out_requests_leave_others_intact(gpu):
+    """A NaN exponent, an f shell and a zero time limit end those requests
+    with an error; the requests beside them give the same result as alone."""
+    good = [serving.Request(0, 8, 51, "eri", 0.0), serving.Request(1, 6, 52, "deriv", 0.0)]
+    alone = {}
+    for r in good:
+        srv = serving.Server(hbm_budget=8 << 30, slots=1, tol=1e-10, chunk_doubles=1 << 22)
+        srv.serve([r])
+        assert r.error is None, r.error
+        alone[r.rid] = (r.digest, r.quartets)
+    nan_shells = workloads.config2(natom=2, seed=53)
+    am, xyz, al, co = nan_shells[3]
+    nan_shells[3] = (am, xyz, [float("nan")] + list(al[1:]), co)
+    f_shells = workloads.config2(natom=2, seed=54) + [(3, (0.0, 0.0, 0.0), [1.0], [1.0])]
+    mix = [serving.Request(0, 8, 51, "eri", 0.0), serving.Request(1, 6, 52, "deriv", 0.01),
+           serving.Request(2, 2, 0, "eri", 0.0, shells=nan_shells),
+           serving.Request(3, 2, 0, "eri", 0.02, shells=f_shells),
+           serving.Request(4, 10, 55, "eri", 0.0, timeout=0.0)]
+    srv = serving.Server(hbm_budget=8 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 22)
+    srv.serve(mix)
+    assert mix[2].error.startswith("rejected") and "exponent" in mix[2].error
+    assert mix[3].error.startswith("rejected") and "angular momentum" in mix[3].error
+    assert mix[4].error.startswith("timeout")
+    for r in mix[:2]:
+        assert r.error is None, r.error
+        assert (r.digest, r.quartets) == alone[r.rid], r.rid
+
+
+def test_small_request_overtakes_large_one(gpu):
+    """Tiles are taken across requests, least work left first: a small
+    request arriving just after a large one finishes first."""
+    big = serving.Request(0, 24, 61, "eri", 0.0)
+    small = serving.Request(1, 4, 62, "eri", 0.3)
+    srv = serving.Server(hbm_budget=8 << 30, slots=2, tol=1e-10, chunk_doubles=1 << 20)
+    srv.serve([big, small])
+    assert big.error is None and small.error is None
+    print("big", big.t_done - big.arrival, "small", small.t_done - small.arrival, "tiles", srv.tiles_run)
+    assert small.t_done < big.t_done
 
 
 def test_screened_deriv_pass_vs_oracle(gpu):
