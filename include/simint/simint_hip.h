@@ -17,6 +17,8 @@ extern "C" {
 /* Shell pairs of one (am1, am2) resident in device memory, in the order of
  * the source structure (its first nshell12_clip pairs). */
 typedef struct simint_hip_pairs simint_hip_pairs;
+/* A shell set resident in device memory (simint_hip_shells_create below). */
+typedef struct simint_hip_shells simint_hip_shells;
 
 int  simint_hip_pairs_create(struct simint_multi_shellpair const * P, simint_hip_pairs ** out);
 void simint_hip_pairs_free(simint_hip_pairs * p);
@@ -116,11 +118,54 @@ long long simint_hip_compute_eri_diag(simint_hip_pairs const * P, double * d_out
  * d_charges (device, one row of Z, x, y, z per charge).  Angular momentum up
  * to 5.  Queues the kernel on the calling thread's stream; returns npair or
  * a negative SIMINT_ERROR_*. */
-typedef struct simint_hip_shells simint_hip_shells;
 int  simint_hip_shells_create(int nshell, struct simint_shell const * shells, simint_hip_shells ** out);
 void simint_hip_shells_free(simint_hip_shells * s);
 long long simint_hip_compute_osoei(int kind, simint_hip_shells const * S, long long npair, int const * d_pairs,
                                    long long const * d_off, int ncenter, double const * d_charges, double * d_out);
+
+/* Device-side front end (no counterpart in the reference, which fills shell
+ * pairs and screening values on the host, skel/simint/shell/shell.c:425-582,
+ * shell_screen.c:226-270):
+ *   simint_hip_pairs_from_shells -- shell pairs (host list of shell index
+ *     pairs into a device shell set, all of one (am1, am2)) filled by a
+ *     kernel: the values simint_create_multi_shellpair2 computes, the
+ *     fast-Schwarz screening values and their descending sort when
+ *     screen_method is SIMINT_SCREEN_FASTSCHWARZ (SIMINT_SCREEN_NONE also
+ *     accepted);
+ *   simint_hip_pair_bounds -- the Schwarz bound of every pair, the largest
+ *     |(bb|bb)| diagonal element, into d_bound (device, one per pair);
+ *   simint_hip_order_pairs -- the permutation that sorts the pairs by
+ *     primitive count, then bound, both descending (stable), into d_order;
+ *   simint_hip_screen_counts -- for bras with bounds d_bra_bound and a
+ *     bound-sorted (descending) ket group d_ket_bound, the number of kets with
+ *     bound_b bound_k >= screen_tol^2 (with `same`, a bra's own group: at most
+ *     b + 1), into d_counts (device).
+ * The last three queue work on the calling thread's stream. */
+int simint_hip_pairs_from_shells(simint_hip_shells const * S, int npair, int const * pairs, int screen_method,
+                                 simint_hip_pairs ** out);
+long long simint_hip_pair_bounds(simint_hip_pairs const * P, double * d_bound);
+int simint_hip_order_pairs(simint_hip_pairs const * P, double const * d_bound, int * d_order);
+int simint_hip_screen_counts(int nbra, double const * d_bra_bound, int nket, double const * d_ket_bound, int same,
+                             double screen_tol, int * d_counts);
+/* A device pair set's primitive-pair data copied back to host arrays (for
+ * inspection and tests): fields holds 14 x nprim doubles, the arrays x y z
+ * PA_x PA_y PA_z PB_x PB_y PB_z alpha prefac alpha2 beta2 screen one after
+ * another; nprim12 the count per pair; ab the 3 x npair A-B components.  Any
+ * of them may be NULL.  Returns nprim (the primitive pairs held) or a
+ * negative SIMINT_ERROR_*; waits for the copies. */
+long long simint_hip_pairs_export(simint_hip_pairs const * P, double * fields, int * nprim12, double * ab);
+
+/* Cartesian order of the drop-in calls' output (simint_compute_eri,
+ * simint_compute_eri_deriv and the one-electron calls), process-wide.
+ * The reference fixes it when its library is generated (Psi4 or GAMESS,
+ * generator/CMakeLists.txt:9-10); here it is switched at run time:
+ * GAMESS orders d as xx yy zz xy xz yz and f as xxx yyy zzz xxy xxz xyy yyz
+ * xzz yzz xyz, the other AMs as Psi4.  The device-resident calls always
+ * write Psi4 order.  Returns the previous ordering or SIMINT_ERROR_ARGUMENT. */
+#define SIMINT_ORDERING_PSI4   0
+#define SIMINT_ORDERING_GAMESS 1
+int simint_hip_set_ordering(int ordering);
+int simint_hip_ordering(void);
 
 int   simint_hip_synchronize(void);
 void *simint_hip_stream(void);                /* hipStream_t of the calling thread */

@@ -122,7 +122,8 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
     want |= set("eri_%s.hip" % (kernel_name(a[1:], G, "f") if a[0] == "f" else kernel_name(a, G))
                 for a, Gs in lpqs.items() for G in Gs)
     srcs = [s for s in srcs if os.path.basename(s) in want]
-    rt = [os.path.join(CSRC, "runtime.hip"), os.path.join(CSRC, "osoei.hip"), os.path.join(CSRC, "shell.cpp"),
+    rt = [os.path.join(CSRC, "runtime.hip"), os.path.join(CSRC, "osoei.hip"), os.path.join(CSRC, "frontend.hip"),
+          os.path.join(CSRC, "shell.cpp"),
           os.path.join(CSRC, "shell_screen.cpp")]
     reports = {}
     objs = {}

@@ -478,7 +478,9 @@ int oei_one(int kind, const simint_shell *s1, const simint_shell *s2, int ncente
     }
     const int nout = oei_nc(s1->am) * oei_nc(s2->am);
     void *host = nullptr, *din = nullptr, *dout = nullptr;
-    hipError_t e = simint_rt::thread_buffers(im.buf.size(), im.buf.size(), sizeof(double) * nout, &host, &din, &dout);
+    // room for the block and its copy in the drop-in calls' Cartesian order
+    hipError_t e = simint_rt::thread_buffers(im.buf.size(), im.buf.size(), 2 * sizeof(double) * nout, &host, &din,
+                                             &dout);
     if (e != hipSuccess) { simint_rt::set_hip_error("one-electron buffers", e); return SIMINT_ERROR_HIP; }
     const hipStream_t s = simint_rt::stream();
     memcpy(host, im.buf.data(), im.buf.size());
@@ -500,7 +502,16 @@ int oei_one(int kind, const simint_shell *s1, const simint_shell *s2, int ncente
         hipStreamSynchronize(s);
         return SIMINT_ERROR_HIP;
     }
-    if ((e = hipMemcpyAsync(integrals, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+    const double *result = static_cast<double *>(dout);
+    bool moved = false;
+    if ((e = simint_rt::to_drop_in_order(s1->am, s2->am, result, static_cast<double *>(dout) + nout, s, &moved)) !=
+        hipSuccess) {
+        simint_rt::set_hip_error("ordering permutation", e);
+        hipStreamSynchronize(s);
+        return SIMINT_ERROR_HIP;
+    }
+    if (moved) result += nout;
+    if ((e = hipMemcpyAsync(integrals, result, sizeof(double) * nout, hipMemcpyDeviceToHost, s)) != hipSuccess) {
         simint_rt::set_hip_error("hipMemcpyAsync(D2H)", e);
         hipStreamSynchronize(s);
         return SIMINT_ERROR_HIP;
@@ -513,13 +524,6 @@ int oei_one(int kind, const simint_shell *s1, const simint_shell *s2, int ncente
 }
 
 }  // namespace
-
-struct simint_hip_shells {
-    int n = 0;
-    int maxam = 0;
-    void *dev = nullptr;
-    OeiArgs base{};
-};
 
 extern "C" {
 
@@ -584,7 +588,19 @@ int simint_hip_shells_create(int nshell, const simint_shell *shells, simint_hip_
     s->n = nshell;
     s->maxam = im.maxam;
     s->dev = d;
-    s->base = shell_args(static_cast<char *>(d), im);
+    const OeiArgs sa = shell_args(static_cast<char *>(d), im);
+    s->host_am = new int[std::max(nshell, 1)];
+    s->host_nprim = new int[std::max(nshell, 1)];
+    for (int i = 0; i < nshell; i++) {
+        s->host_am[i] = shells[i].am;
+        s->host_nprim[i] = shells[i].nprim;
+    }
+    s->am = sa.am;
+    s->nprim = sa.nprim;
+    s->poff = sa.poff;
+    s->xyz = sa.xyz;
+    s->alpha = sa.alpha;
+    s->coef = sa.coef;
     *out = s;
     return 0;
 }
@@ -593,6 +609,8 @@ void simint_hip_shells_free(simint_hip_shells *s)
 {
     if (!s) return;
     if (s->dev) hipFree(s->dev);
+    delete[] s->host_am;
+    delete[] s->host_nprim;
     delete s;
 }
 
@@ -604,7 +622,13 @@ long long simint_hip_compute_osoei(int kind, const simint_hip_shells *S, long lo
     if (kind == 2 && ncenter > 0 && !d_charges) return SIMINT_ERROR_ARGUMENT;
     if (!simint_rt::ensure_device()) return SIMINT_ERROR_NO_DEVICE;
     if (npair == 0) return 0;
-    OeiArgs a = S->base;
+    OeiArgs a{};
+    a.am = S->am;
+    a.nprim = S->nprim;
+    a.poff = S->poff;
+    a.xyz = S->xyz;
+    a.alpha = S->alpha;
+    a.coef = S->coef;
     a.pairs = d_pairs;
     a.off = d_off;
     a.npair = npair;

@@ -552,12 +552,91 @@ hipError_t launch_deriv1_assemble(Deriv1Args d, hipStream_t s)
     return hipGetLastError();
 }
 
+// Cartesian order of the drop-in calls' output.  GAMESS orders d as
+// xx yy zz xy xz yz and f as xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz
+// (generator/Ordering_GAMESS.cpp); every other AM is the same as Psi4's.
+std::atomic<int> g_ordering{SIMINT_ORDERING_PSI4};
+
+// gamess_pos[l][k]: position in GAMESS order of the Psi4 component k of AM l
+__constant__ signed char kGamessPos[4][10] = {
+    {0},
+    {0, 1, 2},
+    // psi4: xx xy xz yy yz zz
+    {0, 3, 4, 1, 5, 2},
+    // psi4: xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz
+    {0, 3, 4, 5, 9, 7, 1, 6, 8, 2},
+};
+
+struct PermArgs {
+    int am[4];
+    int n[4];
+    long long nq;      // quartets
+    int width;         // values per Cartesian quartet (12 for first derivatives)
+    const double *in;
+    double *out;
+};
+
+// out = in with every quartet's Cartesian components moved to GAMESS order
+__global__ void __launch_bounds__(256) to_gamess(const PermArgs p)
+{
+    const long long nc = (long long)p.n[0] * p.n[1] * p.n[2] * p.n[3];
+    const long long total = p.nq * nc * p.width;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+        const int x = (int)(i % p.width);
+        long long r = i / p.width;
+        int comp[4];
+        for (int k = 3; k >= 0; k--) {
+            comp[k] = (int)(r % p.n[k]);
+            r /= p.n[k];
+        }
+        const long long q = r;
+        long long t = 0;
+        for (int k = 0; k < 4; k++) {
+            const int g = p.am[k] <= 3 ? kGamessPos[p.am[k]][comp[k]] : comp[k];
+            t = t * p.n[k] + g;
+        }
+        p.out[(q * nc + t) * p.width + x] = p.in[i];
+    }
+}
+
+hipError_t launch_to_gamess(const int am[4], long long nq, int width, const double *in, double *out, hipStream_t s)
+{
+    PermArgs p;
+    for (int k = 0; k < 4; k++) {
+        p.am[k] = am[k];
+        p.n[k] = ncart(am[k]);
+    }
+    p.nq = nq;
+    p.width = width;
+    p.in = in;
+    p.out = out;
+    const long long total = nq * p.n[0] * p.n[1] * p.n[2] * p.n[3] * width;
+    if (total == 0) return hipSuccess;
+    const long long nb = std::min<long long>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(to_gamess, dim3((unsigned)nb), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+bool gamess_needed(const int am[4])
+{
+    return g_ordering.load(std::memory_order_relaxed) == SIMINT_ORDERING_GAMESS &&
+           (am[0] >= 2 || am[1] >= 2 || am[2] >= 2 || am[3] >= 2);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C API
 extern "C" {
 
 void simint_init(void) { ensure_device(); }
+
+int simint_hip_set_ordering(int ordering)
+{
+    if (ordering != SIMINT_ORDERING_PSI4 && ordering != SIMINT_ORDERING_GAMESS) return SIMINT_ERROR_ARGUMENT;
+    return g_ordering.exchange(ordering);
+}
+
+int simint_hip_ordering(void) { return g_ordering.load(); }
 
 void simint_finalize(void)
 {
@@ -670,7 +749,8 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
     // may still read c.stage, c.in or c.split: wait for it before reuse
     if ((e = hipStreamSynchronize(c.s)) != hipSuccess) { set_err("hipStreamSynchronize", e); return SIMINT_ERROR_HIP; }
     const size_t final_n = (size_t)nP * nQ * nc * (deriv ? 12 : 1);
-    const size_t out_bytes = sizeof(double) * (scratch + (deriv ? final_n : 0));
+    const bool gamess = gamess_needed(am);
+    const size_t out_bytes = sizeof(double) * (scratch + (deriv ? final_n : 0) + (gamess ? final_n : 0));
     if ((e = c.stage.reserve(im.buf.size())) != hipSuccess) { set_err("hipHostMalloc", e); return SIMINT_ERROR_HIP; }
     if ((e = c.in.reserve(im.buf.size())) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
     if ((e = c.out.reserve(out_bytes)) != hipSuccess) { set_err("hipMalloc", e); return SIMINT_ERROR_HIP; }
@@ -732,6 +812,15 @@ int simint_compute_ostei_deriv(int deriv, const simint_multi_shellpair *P, const
         }
         result = d.out;
     }
+    if (gamess) {
+        double *perm = dout + scratch + (deriv ? final_n : 0);
+        if ((e = launch_to_gamess(am, (long long)nP * nQ, deriv ? 12 : 1, result, perm, c.s)) != hipSuccess) {
+            set_err("ordering permutation", e);
+            hipStreamSynchronize(c.s);
+            return SIMINT_ERROR_HIP;
+        }
+        result = perm;
+    }
     e = hipMemcpyAsync(integrals, result, sizeof(double) * final_n, hipMemcpyDeviceToHost, c.s);
     if (e != hipSuccess) {
         set_err("hipMemcpyAsync(D2H)", e);
@@ -759,19 +848,6 @@ size_t simint_eri_worksize(int derorder, int maxam) { return simint_ostei_worksi
 size_t simint_eri_workmem(int derorder, int maxam) { return simint_ostei_workmem(derorder, maxam); }
 
 // --------------------------------------------------------- device-resident
-struct simint_hip_pairs {
-    int am1, am2, n;
-    bool has_screen;
-    double screen_max;
-    void *dev = nullptr;
-    PairsLinear lin;
-    PairsTiled til;
-    int ntile;
-    size_t bytes = 0;                            // device memory held
-    const double *wa = nullptr, *wb = nullptr;   // linear prefac * alpha2, prefac * beta2
-    const double *twa = nullptr, *twb = nullptr; // the same in the tiled view
-};
-
 int simint_hip_pairs_create(const simint_multi_shellpair *P, simint_hip_pairs **out)
 {
     int v = validate(P);
@@ -1375,6 +1451,14 @@ hipError_t thread_buffers(size_t host_bytes, size_t in_bytes, size_t out_bytes, 
     *din = c.in.p;
     *dout = c.out.p;
     return hipSuccess;
+}
+
+hipError_t to_drop_in_order(int la, int lb, const double *in, double *out, hipStream_t s, bool *moved)
+{
+    const int am[4] = {la, lb, 0, 0};
+    *moved = gamess_needed(am);
+    if (!*moved) return hipSuccess;
+    return launch_to_gamess(am, 1, 1, in, out, s);
 }
 
 void set_error(const char *msg) { t_err = msg; }

@@ -37,15 +37,22 @@ class Launch:
 
 
 class Group:
-    """One (pair class, primitive count) group of unique pairs, sorted by bound."""
+    """One (pair class, primitive count) group of unique pairs, sorted by bound.
+    With a device shell set the pairs are filled on the device and the
+    bounds stay there too (dbounds)."""
 
-    def __init__(self, basis, cls, nprim, pairs, bounds):
+    def __init__(self, basis, cls, nprim, pairs, bounds, dshells=None, dbounds=None):
         self.cls = cls
         self.nprim = nprim
         self.pairs = pairs
         self.bounds = bounds
-        self.host = simint.Pairs.from_pairs(basis, pairs)
-        self.dev = simint.DevicePairs(self.host)
+        self.dbounds = dbounds
+        if dshells is None:
+            self.host = simint.Pairs.from_pairs(basis, pairs)
+            self.dev = simint.DevicePairs(self.host)
+        else:
+            self.host = None
+            self.dev = simint.DevicePairs.from_shells(dshells, pairs, cls)
 
 
 def _class_flops(info, la, lb, lc, ld):
@@ -83,11 +90,20 @@ class ScreenedSet:
     mixed: base threshold (see mixed_threshold) below which a quartet's
     bound sends it to the FP32-VRR kernels (None: every quartet FP64)."""
 
-    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None, deriv=0):
+    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None, deriv=0, front_end="host"):
+        """front_end "host": shell pairs filled on the host (the drop-in
+        simint_create_multi_shellpair2) and uploaded, bounds sorted and
+        screened quartets counted in numpy; "device": pairs filled, bounds
+        computed and sorted, and counts taken on the GPU from a device copy
+        of the shells (only the shell-index lists go up and the per-bra
+        counts come back)."""
+        import time
         import torch
+        t0 = time.perf_counter()
         self.tol = tol
         self.mixed = mixed
         self.deriv = deriv
+        self.front_end = front_end
         self.basis = simint.Basis(shells)
         self.shells = shells
         self.out_doubles = out_doubles
@@ -101,7 +117,25 @@ class ScreenedSet:
                 byclass.setdefault((shells[a][0], shells[b][0]), []).append((a, b))
         # Schwarz bounds from diagonal quartets on the GPU
         self.groups = []
-        for cls in sorted(byclass):
+        self.dshells = simint.DeviceShells(self.basis) if front_end == "device" else None
+        for cls in (sorted(byclass) if front_end == "device" else []):
+            pairs = np.array(byclass[cls], dtype=np.int32)
+            P = simint.DevicePairs.from_shells(self.dshells, pairs, cls)
+            bound = torch.empty(len(pairs), dtype=torch.float64, device="cuda")
+            order = torch.empty(len(pairs), dtype=torch.int32, device="cuda")
+            P.bounds(bound.data_ptr())
+            P.order(bound.data_ptr(), order.data_ptr())
+            simint.synchronize()
+            P.free()
+            sb = bound.index_select(0, order.long())
+            ordh = order.cpu().numpy()
+            sp = pairs[ordh]
+            npr = np.array([self._pair_nprim(shells, int(a), int(b)) for a, b in sp], dtype=np.int64)
+            edges = np.flatnonzero(np.diff(npr)) + 1
+            for s0, s1 in zip(np.r_[0, edges], np.r_[edges, len(sp)]):
+                self.groups.append(Group(self.basis, cls, int(npr[s0]), [tuple(int(x) for x in q) for q in sp[s0:s1]],
+                                         sb[s0:s1].cpu().numpy(), self.dshells, sb[s0:s1].contiguous()))
+        for cls in (sorted(byclass) if front_end != "device" else []):
             pairs = byclass[cls]
             tmp = simint.Pairs.from_pairs(self.basis, pairs)
             dev = simint.DevicePairs(tmp)
@@ -121,12 +155,37 @@ class ScreenedSet:
         import os
         here = os.path.dirname(os.path.abspath(__file__))
         self.info = json.load(open(os.path.join(here, "csrc", "gen", "classes.json")))
+        self.setup_pairs_s = time.perf_counter() - t0
         self._plan()
+        self.setup_s = time.perf_counter() - t0
+
+    @staticmethod
+    def _pair_nprim(shells, a, b):
+        na, nb = len(shells[a][2]), len(shells[b][2])
+        return na * (na + 1) // 2 if a == b else na * nb
+
+    def _counts(self, gb, gk, tol):
+        """Per bra of gb: kets of gk (bound-sorted) with bound_b bound_k >=
+        tol^2 (within one group at most b + 1)."""
+        if self.front_end == "device":
+            import torch
+            out = torch.empty(len(gb.pairs), dtype=torch.int32, device="cuda")
+            simint.screen_counts(len(gb.pairs), gb.dbounds.data_ptr(), len(gk.pairs), gk.dbounds.data_ptr(),
+                                 gk is gb, tol, out.data_ptr())
+            simint.synchronize()
+            return out.cpu().numpy().astype(np.int64)
+        kb = gk.bounds
+        with np.errstate(divide="ignore", over="ignore"):
+            need = np.where(gb.bounds > 0, tol * tol / gb.bounds, np.inf)
+        # kb is descending: count of kb >= need
+        n = np.searchsorted(-kb, -need, side="right").astype(np.int64)
+        if gk is gb:
+            n = np.minimum(n, np.arange(1, len(gb.bounds) + 1))
+        return n
 
     def _plan(self):
         """Per (bra group, ket group): prefix lengths for every bra, chunked
         so that one launch's output fits the output buffer."""
-        tol2 = self.tol * self.tol
         self.launches = []
         self.nchunk = 0
         self.total_quartets = 0
@@ -141,15 +200,12 @@ class ScreenedSet:
                 if self.deriv:
                     nc *= 12
                 # kets surviving the screen: bound_b * bound_k >= tol^2
-                kb = gk.bounds
-                with np.errstate(divide="ignore"):
-                    need = np.where(gb.bounds > 0, tol2 / gb.bounds, np.inf)
-                # kb is descending: count of kb >= need
-                nket = np.searchsorted(-kb, -need, side="right").astype(np.int64)
-                if gk is gb:
-                    nket = np.minimum(nket, np.arange(1, len(gb.bounds) + 1))
+                nket = self._counts(gb, gk, self.tol)
                 K = gb.nprim * gk.nprim
                 cls = (la, lb, lc, ld)
+                # kets whose bound keeps them in FP64 on a mixed plan
+                n64 = None if self.mixed is None else self._counts(gb, gk, mixed_threshold(cls, self.mixed))
+                kb = n64
                 cap = self.out_doubles
                 # chunks: runs of whole bras whose output fits `cap`; a bra
                 # whose own kets exceed it is cut into ket ranges
@@ -184,11 +240,8 @@ class ScreenedSet:
             parts = [(lo, hi, off, 0)]
         else:
             # kets are bound-sorted: FP64 for the leading kets with
-            # bound_b * bound_k >= threshold^2
-            thr = mixed_threshold(cls, self.mixed)
-            with np.errstate(divide="ignore", over="ignore"):
-                need64 = np.where(gb.bounds[b:e] > 0, thr ** 2 / gb.bounds[b:e], np.inf)
-            n64 = np.clip(np.searchsorted(-kb, -need64, side="right"), lo0, hi)
+            # bound_b * bound_k >= threshold^2 (their count per bra in kb)
+            n64 = np.clip(kb[b:e], lo0, hi)
             parts = [(lo, n64, off, 0), (n64, hi, off + (n64 - lo0) * nc, 1)]
         for plo, phi, oo, prec in parts:
             cnt = phi - (plo if plo is not None else 0)
@@ -316,5 +369,9 @@ class ScreenedSet:
     def free(self):
         for g in self.groups:
             g.dev.free()
-            g.host.free()
+            if g.host is not None:
+                g.host.free()
+        if self.dshells is not None:
+            self.dshells.free()
+            self.dshells = None
         self.basis.free()

@@ -65,7 +65,11 @@ EXPORTS = [
     "simint_compute_osoei_overlap", "simint_compute_osoei_ke", "simint_compute_osoei_potential",
     "simint_hip_shells_create", "simint_hip_shells_free", "simint_hip_compute_osoei",
     "simint_hip_compute_eri_deriv1_list_noalloc", "simint_hip_pairs_bytes",
+    "simint_hip_set_ordering", "simint_hip_ordering",
+    "simint_hip_pairs_from_shells", "simint_hip_pair_bounds", "simint_hip_order_pairs", "simint_hip_screen_counts",
+    "simint_hip_pairs_export",
 ]
+ORDERING_PSI4, ORDERING_GAMESS = 0, 1
 
 _LIB = None
 
@@ -126,6 +130,7 @@ def lib():
         L.simint_hip_stream.restype = c_void_p
         L.simint_hip_set_stream.argtypes = [c_void_p]
         L.simint_hip_set_list_split.argtypes = [c_int]
+        L.simint_hip_set_ordering.argtypes = [c_int]
         L.simint_hip_last_error.restype = ctypes.c_char_p
         L.simint_hip_class_kernel.argtypes = [c_int] * 4
         L.simint_hip_set_class_kernel.argtypes = [c_int] * 5
@@ -137,6 +142,13 @@ def lib():
         for f in ("simint_compute_potential", "simint_compute_osoei_potential"):
             getattr(L, f).argtypes = [c_int, dptr, dptr, dptr, dptr, SP, SP, dptr]
         L.simint_hip_shells_create.argtypes = [c_int, SP, ctypes.POINTER(c_void_p)]
+        L.simint_hip_pairs_from_shells.argtypes = [c_void_p, c_int, iptr, c_int, ctypes.POINTER(c_void_p)]
+        L.simint_hip_pair_bounds.argtypes = [c_void_p, c_void_p]
+        L.simint_hip_pair_bounds.restype = ll
+        L.simint_hip_order_pairs.argtypes = [c_void_p, c_void_p, c_void_p]
+        L.simint_hip_screen_counts.argtypes = [c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p]
+        L.simint_hip_pairs_export.argtypes = [c_void_p, dptr, iptr, dptr]
+        L.simint_hip_pairs_export.restype = ll
         L.simint_hip_shells_free.argtypes = [c_void_p]
         L.simint_hip_compute_osoei.argtypes = [c_int, c_void_p, ll, c_void_p, c_void_p, c_int, c_void_p, c_void_p]
         L.simint_hip_compute_osoei.restype = ll
@@ -259,6 +271,53 @@ class DevicePairs:
         self.am = (pairs.mp.am1, pairs.mp.am2)
         self.n = pairs.mp.nshell12_clip
         self.bytes = int(lib().simint_hip_pairs_bytes(h))
+
+    @classmethod
+    def from_shells(cls, dshells, pairs, am, screen=SCREEN_NONE):
+        """simint_hip_pairs_from_shells: the pairs (i, j) of a DeviceShells
+        set, filled on the device (all of class `am`)."""
+        pr = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+        h = c_void_p()
+        rc = lib().simint_hip_pairs_from_shells(dshells.h, len(pr), pr.ctypes.data_as(iptr), int(screen),
+                                                ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError("simint_hip_pairs_from_shells failed (%d): %s" % (rc, last_error()))
+        self = cls.__new__(cls)
+        self.h = h
+        self.am = tuple(am)
+        self.n = len(pr)
+        self.bytes = int(lib().simint_hip_pairs_bytes(h))
+        return self
+
+    def export(self):
+        """The linear view copied back: dict of the 14 per-primitive fields
+        (x .. beta2, screen), nprim12 and AB."""
+        L = lib()
+        m = L.simint_hip_pairs_export(self.h, None, None, None)
+        if m < 0:
+            raise RuntimeError("simint_hip_pairs_export failed (%d): %s" % (m, last_error()))
+        f = np.zeros(14 * max(m, 1))
+        npr = np.zeros(max(self.n, 1), dtype=np.int32)
+        ab = np.zeros(3 * max(self.n, 1))
+        r = L.simint_hip_pairs_export(self.h, f.ctypes.data_as(dptr), npr.ctypes.data_as(iptr), ab.ctypes.data_as(dptr))
+        if r < 0:
+            raise RuntimeError("simint_hip_pairs_export failed (%d): %s" % (r, last_error()))
+        names = ["x", "y", "z", "PA_x", "PA_y", "PA_z", "PB_x", "PB_y", "PB_z", "alpha", "prefac", "alpha2",
+                 "beta2", "screen"]
+        d = {k: f[i * m:(i + 1) * m] for i, k in enumerate(names)}
+        d["nprim12"] = npr[:self.n]
+        d["AB"] = ab[:3 * self.n].reshape(3, self.n)
+        return d
+
+    def bounds(self, d_bound):
+        r = lib().simint_hip_pair_bounds(self.h, c_void_p(d_bound))
+        if r < 0:
+            raise RuntimeError("simint_hip_pair_bounds failed (%d): %s" % (r, last_error()))
+
+    def order(self, d_bound, d_order):
+        r = lib().simint_hip_order_pairs(self.h, c_void_p(d_bound), c_void_p(d_order))
+        if r < 0:
+            raise RuntimeError("simint_hip_order_pairs failed (%d): %s" % (r, last_error()))
 
     def free(self):
         if self.h:
@@ -441,6 +500,13 @@ class DeviceShells:
         if self.h:
             lib().simint_hip_shells_free(self.h)
             self.h = None
+
+
+def screen_counts(nbra, d_bra_bound, nket, d_ket_bound, same, tol, d_counts):
+    r = lib().simint_hip_screen_counts(int(nbra), c_void_p(d_bra_bound), int(nket), c_void_p(d_ket_bound),
+                                       1 if same else 0, float(tol), c_void_p(d_counts))
+    if r < 0:
+        raise RuntimeError("simint_hip_screen_counts failed (%d): %s" % (r, last_error()))
 
 
 def synchronize():

@@ -114,12 +114,14 @@ def test_malformed_and_timed_out_requests_leave_others_intact(gpu):
 def test_small_request_overtakes_large_one(gpu):
     """Tiles are taken across requests, least work left first: a small
     request arriving just after a large one finishes first."""
-    big = serving.Request(0, 24, 61, "eri", 0.0)
-    small = serving.Request(1, 4, 62, "eri", 0.3)
+    big = serving.Request(0, 48, 61, "eri", 0.0)
+    small = serving.Request(1, 4, 62, "eri", 0.05)
     srv = serving.Server(hbm_budget=8 << 30, slots=2, tol=1e-10, chunk_doubles=1 << 20)
     srv.serve([big, small])
     assert big.error is None and small.error is None
     print("big", big.t_done - big.arrival, "small", small.t_done - small.arrival, "tiles", srv.tiles_run)
+    # the small one arrived while the big one was still being served
+    assert small.arrival < big.t_done
     assert small.t_done < big.t_done
 
 

@@ -24,6 +24,7 @@ class _Group:
 def _planned(groups, out_doubles, mixed=None, deriv=0, tol=1e-12):
     ss = quartets.ScreenedSet.__new__(quartets.ScreenedSet)
     ss.tol, ss.mixed, ss.deriv, ss.out_doubles, ss.groups = tol, mixed, deriv, out_doubles, groups
+    ss.front_end = "host"
     ss.info = json.load(open(INFO))
     ss._plan()
     return ss
