@@ -56,8 +56,12 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     nodes = g.node_list if keep is None else [n for n in g.node_list if (n.q, n.m) in keep]
     if acc_tops is None:
         acc_tops = [("acc%d" % t, key) for t, key in enumerate(g.top_keys)]
-    else:
-        Ltot = max(n.m for n in nodes if n.step is None)
+    # The Boys values come from one evaluation at the class's total AM even
+    # when a group of tops needs fewer orders, as in the reference (one
+    # boys_F_split per primitive quartet): for x >= 36.5 the lower orders
+    # then carry the same error pattern in every group, which the HRR
+    # transfers between tops of different groups cancel -- with per-group
+    # orders they did not, and (df|dd) lost 1e-7 normwise.
     used_steps = set(n.step for n in nodes if n.step is not None)
     has_bra = bool(used_steps & {I, J})
     has_ket = bool(used_steps & {K, L})

@@ -78,7 +78,13 @@ def test_device_front_end_screened_set(gpu):
     hb = {p: b for g in host.groups for p, b in zip(g.pairs, g.bounds)}
     db = {p: b for g in dev.groups for p, b in zip(g.pairs, g.bounds)}
     assert hb.keys() == db.keys()
-    assert max(abs(hb[p] - db[p]) / max(hb[p], 1e-300) for p in hb) <= 1e-13
+    # the two fills differ in the last bit or two of the prefactor (exp on
+    # the device and on the host); the HRR of far-apart pairs with tight
+    # exponents amplifies that by up to ~1e4 (measured 4e-12 relative on
+    # (dp|dp) diagonals), far below what moves a screening decision
+    rel = sorted(((abs(hb[p] - db[p]) / max(hb[p], 1e-300), p, hb[p], db[p]) for p in hb), reverse=True)
+    print("worst bound differences", rel[:6])
+    assert rel[0][0] <= 1e-10
     for g in dev.groups:
         assert np.all(np.diff(g.bounds) <= 0)
     host.free()
