@@ -155,6 +155,40 @@ int simint_hip_screen_counts(int nbra, double const * d_bra_bound, int nket, dou
  * negative SIMINT_ERROR_*; waits for the copies. */
 long long simint_hip_pairs_export(simint_hip_pairs const * P, double * fields, int * nprim12, double * ab);
 
+/* Coulomb and exchange digestion (a Fock build's consumer of the integrals;
+ * no counterpart in the reference, which returns integrals only).
+ * simint_hip_compute_jk_range computes the quartets of one range launch as
+ * simint_hip_compute_eri_range does (integrals into d_out) and then, on the
+ * same stream, contracts them with the density:
+ *     J_mn = sum_ls (mn|ls) D_ls,     K_mn = sum_ls (ml|ns) D_ls,
+ * counting each quartet with the weight of its permutational images, so
+ * that digesting every unique shell quartet (ab|cd) -- a >= b, c >= d,
+ * (ab) >= (cd) in any fixed pair order -- exactly once and then calling
+ * simint_hip_jk_symmetrize gives J and K over all basis functions.
+ * jk->bra_fun / ket_fun (device, 2 ints per pair of P / Q): the first basis
+ * function of each shell of the pair; two shells are the same shell when
+ * their first functions are equal.  jk->same != 0: P and Q are one pair set
+ * and quartet (b, b) is its own bra-ket image.  density, J, K: nbf x nbf,
+ * row-major, density symmetric; J and K accumulate (atomically, so the
+ * summation order and the last bits vary between runs); either may be
+ * NULL.  Returns the quartets digested or a negative SIMINT_ERROR_*. */
+typedef struct simint_hip_jk {
+    int nbf;
+    int same;
+    const int *bra_fun;
+    const int *ket_fun;
+    const double *density;
+    double *J;
+    double *K;
+} simint_hip_jk;
+long long simint_hip_compute_jk_range(simint_hip_pairs const * P, simint_hip_pairs const * Q, int b0, int nbra,
+                                      int const * ket_lo, int const * ket_hi, long long const * out_off,
+                                      double screen_tol, int precision, double * d_out,
+                                      simint_hip_jk const * jk);
+/* J <- (J + J^T) / 2 and K <- (K + K^T) / 2 (device, nbf x nbf, either may
+ * be NULL), on the calling thread's stream. */
+int simint_hip_jk_symmetrize(int nbf, double * d_J, double * d_K);
+
 /* Cartesian order of the drop-in calls' output (simint_compute_eri,
  * simint_compute_eri_deriv and the one-electron calls), process-wide.
  * The reference fixes it when its library is generated (Psi4 or GAMESS,

@@ -227,3 +227,31 @@ def md_oei(sa, pairs, kind, Z=None, cxyz=None, reference_boys=False):
         return _oei(lib().md_oei_pairs, kind, sa, pairs, Z, cxyz)
     finally:
         lib().md_set_reference_boys(0)
+
+
+def eri_tensor(sa, exact=False):
+    """The full four-index tensor (mu nu|lambda sigma) over the Cartesian
+    functions of every shell (Psi4 order within a shell, shells in input
+    order), from every ordered shell quartet -- no permutational symmetry
+    used.  exact=True: the long-double McMurchie-Davidson values."""
+    ns = len(sa.am)
+    nf = [ncart(int(l)) for l in sa.am]
+    f0 = np.zeros(ns + 1, dtype=np.int64)
+    f0[1:] = np.cumsum(nf)
+    nbf = int(f0[-1])
+    quartets = [(a, b, c, d) for a in range(ns) for b in range(ns) for c in range(ns) for d in range(ns)]
+    vals, off, n = (md_eri(sa, quartets, reference_boys=True) if exact else eri(sa, quartets))
+    g = np.zeros((nbf,) * 4)
+    for (a, b, c, d), o, m in zip(quartets, off, n):
+        g[f0[a]:f0[a + 1], f0[b]:f0[b + 1], f0[c]:f0[c + 1], f0[d]:f0[d + 1]] = \
+            vals[o:o + m].reshape(nf[a], nf[b], nf[c], nf[d])
+    return g, f0
+
+
+def jk(g, D):
+    """Coulomb and exchange matrices of a density D from the full tensor:
+    J_mn = sum_ls (mn|ls) D_ls, K_mn = sum_ls (ml|ns) D_ls (the digestion a
+    Fock build performs; the reference has no such consumer)."""
+    J = np.einsum("mnls,ls->mn", g, D)
+    K = np.einsum("mlns,ls->mn", g, D)
+    return J, K

@@ -123,6 +123,7 @@ def build(maxam=DEFAULT_MAXAM, jobs=None, verbose=True):
                 for a, Gs in lpqs.items() for G in Gs)
     srcs = [s for s in srcs if os.path.basename(s) in want]
     rt = [os.path.join(CSRC, "runtime.hip"), os.path.join(CSRC, "osoei.hip"), os.path.join(CSRC, "frontend.hip"),
+          os.path.join(CSRC, "jk.hip"),
           os.path.join(CSRC, "shell.cpp"),
           os.path.join(CSRC, "shell_screen.cpp")]
     reports = {}

@@ -568,3 +568,31 @@ long long simint_hip_pairs_export(const simint_hip_pairs *P, double *fields, int
 }
 
 }  // extern "C"
+
+namespace simint_rt {
+
+int host_pair_bounds(const simint_multi_shellpair *P, double *out)
+{
+    simint_hip_pairs *dp = nullptr;
+    int rc = simint_hip_pairs_create(P, &dp);
+    if (rc < 0) return rc;
+    double *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(double) * (P->nshell12 > 0 ? P->nshell12 : 1));
+    if (e != hipSuccess) {
+        simint_hip_pairs_free(dp);
+        set_hip_error("hipMalloc(pair bounds)", e);
+        return SIMINT_ERROR_HIP;
+    }
+    const long long r = simint_hip_pair_bounds(dp, d);   // synchronises the stream
+    if (r >= 0 && P->nshell12 > 0) {
+        e = hipMemcpyAsync(out, d, sizeof(double) * P->nshell12, hipMemcpyDeviceToHost, stream());
+        if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    }
+    hipFree(d);
+    simint_hip_pairs_free(dp);
+    if (r < 0) return (int)r;
+    if (e != hipSuccess) { set_hip_error("pair bounds copy", e); return SIMINT_ERROR_HIP; }
+    return 0;
+}
+
+}  // namespace simint_rt

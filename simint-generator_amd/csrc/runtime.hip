@@ -937,10 +937,14 @@ struct RangeTerm {
     int weight;   // -1: none
 };
 
+// items_out / nitems_out (optional): the work items of the first term, left
+// in the thread's item buffer for kernels queued after the class launch
 long long range_launch(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra, const int *ket_lo,
                        const int *ket_hi, const long long *const *out_offs, const RangeTerm *terms, int nterm,
-                       double screen_tol, double *const *d_outs)
+                       double screen_tol, double *const *d_outs, const WorkItem **items_out = nullptr,
+                       long long *nitems_out = nullptr)
 {
+    if (nitems_out) *nitems_out = 0;
     if (nbra == 0) return 0;
     std::vector<long long> tstart(nbra + 1, 0);
     long long nq = 0;
@@ -1014,6 +1018,8 @@ long long range_launch(const simint_hip_pairs *P, const simint_hip_pairs *Q, int
         a.stride = stride;
         if ((e = T.fn(&a, c.s)) != hipSuccess) { set_err("class kernel launch", e); return SIMINT_ERROR_HIP; }
     }
+    if (items_out) *items_out = static_cast<const WorkItem *>(c.items.p);
+    if (nitems_out) *nitems_out = nitems;
     return nq;
 }
 
@@ -1033,6 +1039,28 @@ long long simint_hip_compute_eri_range(const simint_hip_pairs *P, const simint_h
     if (!t.fn) t.fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
     if (!t.fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
     return range_launch(P, Q, b0, nbra, ket_lo, ket_hi, &out_off, &t, 1, screen_tol, &d_out);
+}
+
+long long simint_hip_compute_jk_range(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,
+                                      const int *ket_lo, const int *ket_hi, const long long *out_off,
+                                      double screen_tol, int precision, double *d_out, const simint_hip_jk *jk)
+{
+    if (!jk || jk->nbf <= 0 || !jk->density || !jk->bra_fun || !jk->ket_fun) return SIMINT_ERROR_ARGUMENT;
+    if (!P || !Q || b0 < 0 || nbra < 0 || b0 + nbra > P->n || (precision != 0 && precision != 1))
+        return SIMINT_ERROR_ARGUMENT;
+    if (!ensure_device()) return SIMINT_ERROR_NO_DEVICE;
+    RangeTerm t{{P->am1, P->am2, Q->am1, Q->am2}, nullptr, -1};
+    if (precision == 1) t.fn = simint_dispatch_mixed(P->am1, P->am2, Q->am1, Q->am2);
+    if (!t.fn) t.fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
+    if (!t.fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
+    const WorkItem *items = nullptr;
+    long long nitems = 0;
+    const long long nq = range_launch(P, Q, b0, nbra, ket_lo, ket_hi, &out_off, &t, 1, screen_tol, &d_out, &items,
+                                      &nitems);
+    if (nq <= 0 || nitems == 0 || (!jk->J && !jk->K)) return nq;
+    const hipError_t e = simint_rt::launch_jk_digest(t.am, items, nitems, Q->til.col, d_out, jk, ctx().s);
+    if (e != hipSuccess) { set_err("digest launch", e); return SIMINT_ERROR_HIP; }
+    return nq;
 }
 
 long long simint_hip_compute_eri_prefix(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra,

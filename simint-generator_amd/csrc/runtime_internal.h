@@ -6,6 +6,9 @@
 #include <stddef.h>
 
 #include "device_types.h"
+#include "simint/simint_hip.h"
+
+struct simint_multi_shellpair;
 
 // Shell pairs of one (am1, am2) resident in device memory: the linear and
 // tiled views the class kernels read, and the 2a / 2b weighted prefactors of
@@ -56,5 +59,13 @@ hipError_t thread_buffers(size_t host_bytes, size_t in_bytes, size_t out_bytes, 
 hipError_t to_drop_in_order(int la, int lb, const double *in, double *out, hipStream_t s, bool *moved);
 void set_error(const char *msg);
 void set_hip_error(const char *what, hipError_t e);
+// Largest |(bb|bb)| diagonal element of every shell pair of a host pair set,
+// into out (host, one per pair); 0 or a negative SIMINT_ERROR_*.  Declared
+// again in shell_screen.cpp, which is compiled as plain C++.
+int host_pair_bounds(const simint_multi_shellpair *P, double *out);
+// jk.hip: queue the J/K digestion of the integrals the class kernel of class
+// am wrote for `items` (ket tile lanes mapped to pairs by col) on stream s
+hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long nitems, const long long *col,
+                            const double *ints, const simint_hip_jk *jk, hipStream_t s);
 
 }  // namespace simint_rt

@@ -2,9 +2,10 @@
 //
 // Fast Schwarz is the closed-form (00|00) bound of each primitive pair.
 // Schwarz computes the (ij|ij) integrals of every uncontracted primitive
-// pair on the GPU -- all primitive pairs of one shell pair in a single
-// launch -- and keeps the largest diagonal element (not square-rooted; the
-// compute entry points compare against screen_tol^2).
+// pair on the GPU -- the diagonal quartets of all primitive pairs of one
+// shell pair in a single launch, not every pair against every pair -- and
+// keeps the largest diagonal element (not square-rooted; the compute entry
+// points compare against screen_tol^2).
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -32,31 +33,13 @@ inline double fast_value(const simint_shell *A, const simint_shell *B, int i, in
     return G;
 }
 
-// max |(ab|ab)| diagonal element for each of the npair one-primitive pairs
-// in P (all computed in one call); returns false on a device error
-bool diagonal_maxima(const simint_multi_shellpair *P, int n1, int n2, std::vector<double> &out)
-{
-    const int npair = P->nshell12;
-    const int n12 = n1 * n2;
-    const size_t per = (size_t)n12 * n12;
-    std::vector<double> ints((size_t)npair * npair * per);
-    const int rc = simint_compute_eri(P, P, 0.0, nullptr, ints.data());
-    if (rc < 0) return false;
-    out.assign(npair, 0.0);
-    for (int k = 0; k < npair; k++) {
-        const double *q = ints.data() + ((size_t)k * npair + k) * per;
-        double mx = 0.0;
-        for (int m = 0; m < n1; m++)
-            for (int n = 0; n < n2; n++) {
-                const double v = fabs(q[(size_t)(m * n2 + n) * n12 + (m * n2 + n)]);
-                mx = v > mx ? v : mx;
-            }
-        out[k] = mx;
-    }
-    return true;
-}
-
 }  // namespace
+
+namespace simint_rt {
+// frontend.hip: the largest |(bb|bb)| diagonal element of every pair of P,
+// from the (bb|bb) quartets alone (npair quartets, not npair^2)
+int host_pair_bounds(const simint_multi_shellpair *P, double *out);
+}
 
 extern "C" {
 
@@ -119,8 +102,8 @@ double simint_primscreen_schwarz(const simint_shell *A, const simint_shell *B, d
     simint_multi_shellpair P;
     simint_initialize_multi_shellpair(&P);
     simint_create_multi_shellpair2(npair, pairs.data(), &P, SIMINT_SCREEN_NONE);
-    std::vector<double> mx;
-    const bool ok = diagonal_maxima(&P, ncart(A->am), ncart(B->am), mx);
+    std::vector<double> mx(npair > 0 ? npair : 1);
+    const bool ok = simint_rt::host_pair_bounds(&P, mx.data()) == 0;
     simint_free_multi_shellpair(&P);
     for (auto &s : pairs) simint_free_shell(&s);
     if (!ok) return NAN;

@@ -366,6 +366,38 @@ class ScreenedSet:
         """Device memory the set holds (its shell-pair groups)."""
         return sum(g.dev.bytes for g in self.groups)
 
+    def nbf(self):
+        """Number of Cartesian basis functions of the shells."""
+        return int(sum(simint.ncart(s[0]) for s in self.shells))
+
+    def _group_fun(self, g):
+        """Device array of the first basis function of both shells of every
+        pair of group g (built on first use)."""
+        if getattr(g, "dfun", None) is None:
+            import torch
+            f0 = np.r_[0, np.cumsum([simint.ncart(s[0]) for s in self.shells])].astype(np.int32)
+            fun = f0[np.asarray(g.pairs, dtype=np.int64).reshape(-1)]
+            g.dfun = torch.from_numpy(np.ascontiguousarray(fun, dtype=np.int32)).cuda()
+        return g.dfun
+
+    def run_jk(self, out_ptr, d_density, d_J, d_K, launches=None, events=None):
+        """Queue the launches (default: the whole pass) with the J/K
+        digestion of their quartets after each class kernel: J and K
+        (device nbf x nbf, accumulated) receive every unique quartet of the
+        set once; call simint.jk_symmetrize after the last launch."""
+        if self.deriv:
+            raise ValueError("J/K digestion takes plain integrals, not derivatives")
+        nbf = self.nbf()
+        for ln in (self.launches if launches is None else launches):
+            if events:
+                events(ln, True)
+            jk = simint.JK(nbf, 1 if ln.bra is ln.ket else 0, self._group_fun(ln.bra).data_ptr(),
+                           self._group_fun(ln.ket).data_ptr(), d_density, d_J, d_K)
+            simint.compute_jk_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, jk, 0.0,
+                                    ln.precision)
+            if events:
+                events(ln, False)
+
     def free(self):
         for g in self.groups:
             g.dev.free()

@@ -67,7 +67,7 @@ EXPORTS = [
     "simint_hip_compute_eri_deriv1_list_noalloc", "simint_hip_pairs_bytes",
     "simint_hip_set_ordering", "simint_hip_ordering",
     "simint_hip_pairs_from_shells", "simint_hip_pair_bounds", "simint_hip_order_pairs", "simint_hip_screen_counts",
-    "simint_hip_pairs_export",
+    "simint_hip_pairs_export", "simint_hip_compute_jk_range", "simint_hip_jk_symmetrize",
 ]
 ORDERING_PSI4, ORDERING_GAMESS = 0, 1
 
@@ -149,6 +149,10 @@ def lib():
         L.simint_hip_screen_counts.argtypes = [c_int, c_void_p, c_int, c_void_p, c_int, c_double, c_void_p]
         L.simint_hip_pairs_export.argtypes = [c_void_p, dptr, iptr, dptr]
         L.simint_hip_pairs_export.restype = ll
+        L.simint_hip_compute_jk_range.argtypes = [c_void_p, c_void_p, c_int, c_int, iptr, iptr, llptr, c_double,
+                                                  c_int, c_void_p, ctypes.POINTER(JK)]
+        L.simint_hip_compute_jk_range.restype = ll
+        L.simint_hip_jk_symmetrize.argtypes = [c_int, c_void_p, c_void_p]
         L.simint_hip_shells_free.argtypes = [c_void_p]
         L.simint_hip_compute_osoei.argtypes = [c_int, c_void_p, ll, c_void_p, c_void_p, c_int, c_void_p, c_void_p]
         L.simint_hip_compute_osoei.restype = ll
@@ -422,6 +426,33 @@ def compute_range(P, Q, b0, ket_lo, ket_hi, out_off, d_out_ptr, screen_tol=0.0, 
     if r < 0:
         raise RuntimeError("simint_hip_compute_eri_range failed (%d): %s" % (r, last_error()))
     return r
+
+
+class JK(ctypes.Structure):
+    """simint_hip_jk: the J/K digestion's device arrays."""
+    _fields_ = [("nbf", c_int), ("same", c_int), ("bra_fun", c_void_p), ("ket_fun", c_void_p),
+                ("density", c_void_p), ("J", c_void_p), ("K", c_void_p)]
+
+
+def compute_jk_range(P, Q, b0, ket_lo, ket_hi, out_off, d_out_ptr, jk, screen_tol=0.0, precision=0):
+    """simint_hip_compute_jk_range: compute_range, then the J/K digestion of
+    the same quartets with the density and accumulators of `jk` (a JK)."""
+    ket_hi = np.ascontiguousarray(ket_hi, dtype=np.int32)
+    lo = None if ket_lo is None else np.ascontiguousarray(ket_lo, dtype=np.int32)
+    out_off = np.ascontiguousarray(out_off, dtype=np.int64)
+    r = lib().simint_hip_compute_jk_range(P.h, Q.h, int(b0), len(ket_hi),
+                                          None if lo is None else lo.ctypes.data_as(iptr),
+                                          ket_hi.ctypes.data_as(iptr), out_off.ctypes.data_as(llptr),
+                                          float(screen_tol), int(precision), c_void_p(d_out_ptr), ctypes.byref(jk))
+    if r < 0:
+        raise RuntimeError("simint_hip_compute_jk_range failed (%d): %s" % (r, last_error()))
+    return r
+
+
+def jk_symmetrize(nbf, d_J, d_K):
+    r = lib().simint_hip_jk_symmetrize(int(nbf), c_void_p(d_J or None), c_void_p(d_K or None))
+    if r < 0:
+        raise RuntimeError("simint_hip_jk_symmetrize failed (%d): %s" % (r, last_error()))
 
 
 def compute_deriv1_range(P, Q, b0, ket_lo, ket_hi, d_work, d_out_ptr, screen_tol=0.0):

@@ -101,6 +101,26 @@ def test_all_classes_molecule(gpu, mol):
     print("worst normwise over classes", max(worst.values()))
 
 
+@pytest.mark.parametrize("mol", ["benzene.sto-3g", "ethane.6-31gss", "water.dzp"])
+def test_fixture_molecules_all_classes(gpu, mol):
+    """The other molecules of simint's test fixtures (skel/test/dat/): every
+    class of their shells, P and Q built from the first and last shell of
+    each AM (up to 16 quartets per class), against the oracle and the
+    long-double values."""
+    shells = mol_shells(mol)
+    basis = simint.Basis(shells)
+    sa = normalized_arrays(shells)
+    byam = {}
+    for n, s in enumerate(shells):
+        byam.setdefault(s[0], []).append(n)
+    byam = {l: sorted({v[0], v[-1]}) for l, v in byam.items()}
+    worst = 0.0
+    for i, j, k, l in itertools.product(sorted(byam), repeat=4):
+        st = _check(sa, basis, byam[i], byam[j], byam[k], byam[l], exact=True)
+        worst = max(worst, st["max_normwise"])
+    print(mol, "worst normwise over classes", worst)
+
+
 def _f_shells(seed=11):
     """Two shells of each AM 0..3 on random centres in a 4-bohr cube,
     contraction depth 1-3, exponents log-uniform in [0.1, 50]."""

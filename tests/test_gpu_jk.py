@@ -1,0 +1,118 @@
+"""Coulomb/exchange digestion on the GPU (simint_hip_compute_jk_range and
+simint_hip_jk_symmetrize) against the oracle's J and K from the full
+four-index tensor (every ordered shell quartet, no permutational symmetry
+used, oracle.jk).
+
+The GPU digests each unique shell quartet of a ScreenedSet once (no
+screening: tol 0) with the weight of its permutational images; a wrong
+weight for same-shell pairs, for a quartet that is its own bra-ket image,
+or a wrong image's index pattern moves J or K by O(1) relative to their
+size.  The GPU sums with atomics in varying order, so the bar is normwise:
+max|gpu - oracle| <= 1e-12 max|oracle| (the integrals themselves meet the
+1e-14 bar; J and K are sums of up to nbf^2 terms each), or, where that
+fails, at most 4x the oracle's own normwise distance from J and K of the
+long-double tensor (for f shells with tight exponents the oracle's K is
+itself ~1e-12 away from the exact one)."""
+import numpy as np
+import pytest
+
+import oracle
+import quartets
+import simint
+import workloads
+from helpers import mol_shells, normalized_arrays
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _f_set(seed=3):
+    """Two shells of each AM 0..3 on random centres, contraction depth 1-3."""
+    rng = np.random.default_rng(seed)
+    shells = []
+    for l in range(4):
+        for _ in range(2):
+            k = int(rng.integers(1, 4))
+            shells.append((l, tuple(rng.uniform(0, 3, 3)), list(np.exp(rng.uniform(np.log(0.1), np.log(30), k))),
+                           list(rng.uniform(0.1, 1.0, k))))
+    return shells
+
+
+def _gpu_jk(shells, D, mixed=None, chunk=1 << 22, want=("J", "K")):
+    import torch
+    ss = quartets.ScreenedSet(shells, tol=0.0, out_doubles=chunk, mixed=mixed)
+    try:
+        nbf = ss.nbf()
+        dD = torch.from_numpy(D).cuda()
+        J = torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda")
+        K = torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda")
+        out = torch.empty(chunk, dtype=torch.float64, device="cuda")
+        jp = J.data_ptr() if "J" in want else None
+        kp = K.data_ptr() if "K" in want else None
+        ss.run_jk(out.data_ptr(), dD.data_ptr(), jp, kp)
+        simint.jk_symmetrize(nbf, jp, kp)
+        simint.synchronize()
+        return J.cpu().numpy(), K.cpu().numpy(), ss.total_quartets
+    finally:
+        ss.free()
+
+
+def _density(nbf, seed):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(-1, 1, (nbf, max(2, nbf // 4)))
+    return np.ascontiguousarray(C @ C.T / nbf)
+
+
+def _compare(shells, seed, **kw):
+    sa = normalized_arrays(shells)
+    g, _ = oracle.eri_tensor(sa)
+    D = _density(g.shape[0], seed)
+    Jr, Kr = oracle.jk(g, D)
+    J, K, nq = _gpu_jk(shells, D, **kw)
+    n = len(shells)
+    # unique quartets: pairs i >= j, then pair-of-pairs ij >= kl
+    npair = n * (n + 1) // 2
+    assert nq == npair * (npair + 1) // 2
+    ej = np.abs(J - Jr).max() / np.abs(Jr).max()
+    ek = np.abs(K - Kr).max() / np.abs(Kr).max()
+    print("nbf %d  J normwise %.2e  K normwise %.2e" % (g.shape[0], ej, ek))
+    ok = [ej <= TOL, ek <= TOL]
+    if not all(ok):
+        ge, _ = oracle.eri_tensor(sa, exact=True)
+        for i, (got, ref, ex) in enumerate(zip((J, K), (Jr, Kr), oracle.jk(ge, D))):
+            o = np.abs(ref - ex).max() / np.abs(ex).max()
+            e = np.abs(got - ex).max() / np.abs(ex).max()
+            print("%s vs long double: oracle %.2e gpu %.2e" % ("JK"[i], o, e))
+            ok[i] = ok[i] or e <= max(TOL, 4 * o)
+    return all(ok)
+
+
+def test_jk_water(gpu):
+    """Water 6-31G** (s, p and d shells; same-shell pairs; every quartet that
+    is its own bra-ket image)."""
+    assert _compare(mol_shells("water.6-31gss"), 1)
+
+
+def test_jk_f_shells_small_chunks(gpu):
+    """Shells up to f (split and cooperative class kernels among them) with
+    an output buffer small enough that launches are cut into ket ranges."""
+    assert _compare(_f_set(), 2, chunk=1 << 14)
+
+
+def test_jk_tight_exponents(gpu):
+    """Two atoms of the 64-atom recipe (core exponents up to 5000, depth-6
+    contractions)."""
+    assert _compare(workloads.config2(natom=2, seed=7), 3)
+
+
+def test_jk_only_one_of_them(gpu):
+    """J alone and K alone (the other accumulator NULL) give the same
+    matrices as both together."""
+    shells = mol_shells("water.sto-3g")
+    D = _density(sum(simint.ncart(s[0]) for s in shells), 4)
+    J, K, _ = _gpu_jk(shells, D)
+    J1, _, _ = _gpu_jk(shells, D, want=("J",))
+    _, K1, _ = _gpu_jk(shells, D, want=("K",))
+    assert np.abs(J - J1).max() <= TOL * np.abs(J).max()
+    assert np.abs(K - K1).max() <= TOL * np.abs(K).max()

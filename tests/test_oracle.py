@@ -348,3 +348,81 @@ def test_osoei_s_closed_forms():
     got = [oracle.osoei(sa, [(0, 1)], k, np.array([3.0]), Cc.reshape(1, 3))[0][0] for k in range(3)]
     for g, e in zip(got, (S, T, V)):
         assert abs(g - e) <= 1e-14 * abs(e), (g, e)
+
+
+def _rank_one_case(l, seed):
+    """Shells of mixed AM on several centres, followed by three shells of
+    AM l on one centre; D = c c^T on the x^l components of those three.
+    phi = sum_s c_s chi_s is then the x^l component of one merged contracted
+    shell (appended last to the returned arrays, unnormalised), so
+    J_mn = (mn|phi phi) and K_mn = (m phi|n phi) are single quartets of the
+    long-double brute force, independent of any tensor assembly."""
+    rng = np.random.default_rng(seed)
+    others = _random_shells(seed, lmax=2, per_l=1, depth=(1, 3), box=1.5)
+    cen = tuple(rng.uniform(-1, 1, 3))
+    phis = [(l, cen, list(np.exp(rng.uniform(np.log(0.1), np.log(20), k))), list(rng.uniform(0.2, 1.0, k)))
+            for k in (1, 2, 3)]
+    shells = others + phis
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    c = rng.uniform(-1, 1, len(phis))
+    first = len(others)
+    al = np.concatenate([sa.alpha[sa.off[s]:sa.off[s] + sa.nprim[s]] for s in range(first, len(shells))])
+    co = np.concatenate([c[k] * sa.coef[sa.off[first + k]:sa.off[first + k] + sa.nprim[first + k]]
+                         for k in range(len(phis))])
+    full = oracle.ShellArrays([(int(sa.am[s]), tuple(sa.xyz[s]), list(sa.alpha[sa.off[s]:sa.off[s] + sa.nprim[s]]),
+                                list(sa.coef[sa.off[s]:sa.off[s] + sa.nprim[s]])) for s in range(len(shells))]
+                              + [(l, cen, list(al), list(co))])
+    nf = [oracle.ncart(int(x)) for x in sa.am]
+    f0 = np.r_[0, np.cumsum(nf)]
+    vec = np.zeros(f0[-1])
+    vec[f0[first:-1]] = c
+    return sa, full, np.outer(vec, vec), nf, f0
+
+
+@pytest.mark.parametrize("l", [0, 1, 2])
+def test_jk_rank_one_density(l):
+    """oracle.jk on the full tensor against single long-double quartets:
+    for D = c c^T with phi = sum c_mu chi_mu one merged shell's component,
+    J_mn = (mn|phi phi) and K_mn = (m phi|n phi) for every pair of functions
+    on any centre.  A swapped J/K index pattern, a transposed block in the
+    tensor or a wrong function offset fails this."""
+    sa, full, D, nf, f0 = _rank_one_case(l, 40 + l)
+    g, _ = oracle.eri_tensor(sa)
+    J, K = oracle.jk(g, D)
+    ns, m = len(nf), len(nf)
+    qJ = [(a, b, m, m) for a in range(ns) for b in range(ns)]
+    qK = [(a, m, b, m) for a in range(ns) for b in range(ns)]
+    vJ, oJ, _ = oracle.md_eri(full, qJ, reference_boys=True)
+    vK, oK, _ = oracle.md_eri(full, qK, reference_boys=True)
+    nphi = oracle.ncart(l)
+    Jx = np.zeros_like(J)
+    Kx = np.zeros_like(K)
+    for (a, b, _, _), o in zip(qJ, oJ):
+        Jx[f0[a]:f0[a + 1], f0[b]:f0[b + 1]] = vJ[o:o + nf[a] * nf[b] * nphi * nphi].reshape(
+            nf[a], nf[b], nphi, nphi)[:, :, 0, 0]
+    for (a, _, b, _), o in zip(qK, oK):
+        Kx[f0[a]:f0[a + 1], f0[b]:f0[b + 1]] = vK[o:o + nf[a] * nf[b] * nphi * nphi].reshape(
+            nf[a], nphi, nf[b], nphi)[:, 0, :, 0]
+    assert np.abs(J - Jx).max() <= 1e-12 * np.abs(Jx).max()
+    assert np.abs(K - Kx).max() <= 1e-12 * np.abs(Kx).max()
+
+
+def test_jk_tensor_symmetry_and_psd():
+    """The assembled tensor has the 8-fold permutational symmetry of real
+    integrals (it is built from every ordered quartet, none inferred), and
+    for a random positive semidefinite density over shells on several
+    centres K is positive semidefinite and tr(D J) > 0."""
+    shells = _random_shells(8, lmax=2, per_l=2, depth=(1, 3), box=1.5)
+    sa = oracle.ShellArrays(shells)
+    oracle.normalize(sa)
+    g, f0 = oracle.eri_tensor(sa)
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1)]:
+        assert np.abs(g - g.transpose(perm)).max() <= 1e-13 * np.abs(g).max()
+    rng = np.random.default_rng(2)
+    C = rng.uniform(-1, 1, (g.shape[0], 3))
+    D = C @ C.T
+    J, K = oracle.jk(g, D)
+    assert np.trace(D @ J) > 0
+    assert np.linalg.eigvalsh(0.5 * (K + K.T)).min() >= -1e-12 * np.abs(K).max()
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
