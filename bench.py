@@ -35,7 +35,9 @@ Besides the metric the line carries:
                     server: p50/p99 latency and quartets/s;
   batched           1e6 quartets of all 256 classes up to f in one graph;
   deriv1            first derivatives of the 64-atom system;
-  fp64_64atom       the 64-atom system in FP64 (the all-classes-to-d config);
+  fp64_64atom       the 64-atom system in FP64 (the all-classes-to-d config),
+                    and the same pass with the Coulomb/exchange digestion
+                    after every launch (fp64_64atom.jk);
   cpu_baseline      the CPU oracle (a C restatement of the reference
                     algorithm) on every host core the box gives this job,
                     on a sample of the same quartets.
@@ -283,12 +285,16 @@ def mixed_system_leg(args):
 
 
 def roofline_of(per, info):
-    """Roofline object for the kernel (class, precision) with the largest
-    device time among per-launch timings."""
+    """Roofline object for the kernel with the largest device time among
+    per-launch timings.  Launches are grouped by the kernel they run: the
+    FP32-VRR launches of a class whose selected kernel has no mixed variant
+    run the same FP64 kernel as its FP64 launches and count with them."""
     agg = {}
     for ln, s in per:
-        key = (ln.cls, ln.precision)
-        a = agg.setdefault(key, {"s": 0.0, "f64": 0.0, "f32": 0.0, "bytes": 0.0, "n": 0})
+        key = kernel_symbol(info, ln.cls, ln.precision)
+        a = agg.setdefault(key, {"s": 0.0, "f64": 0.0, "f32": 0.0, "bytes": 0.0, "n": 0, "cls": ln.cls,
+                                 "mixed": False})
+        a["mixed"] = a["mixed"] or (ln.precision == 1 and key.startswith("eri_lpqm_"))
         f64, f32 = launch_split_flops(info, ln)
         a["s"] += s
         a["f64"] += f64
@@ -302,7 +308,7 @@ def roofline_of(per, info):
     f64, f32, by = a["f64"] / a["n"], a["f32"] / a["n"], a["bytes"] / a["n"]
     t_flop = f64 / PEAK_FP64 + f32 / PEAK_FP32
     t_byte = by / PEAK_HBM
-    kname = kernel_symbol(info, key[0], key[1])
+    kname = key
     if t_flop >= t_byte:
         # FP64 vector (VALU) bound: no MFMA is involved; "mfma" is the
         # compute-bound label of the line format
@@ -312,8 +318,8 @@ def roofline_of(per, info):
     else:
         ach = by / mean_s / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / (PEAK_HBM / 1e9)}
-    roof.update({"traffic": measured_traffic(kname), "kernel": kname, "class": cname(key[0]),
-                 "precision": "mixed" if key[1] else "fp64",
+    roof.update({"traffic": measured_traffic(kname), "kernel": kname, "class": cname(a["cls"]),
+                 "precision": "mixed" if a["mixed"] else "fp64",
                  "compute": "vector ALU (no MFMA); compute peak is the FP64 vector rate (78.6 TFLOP/s) with "
                             "FP32 flops priced at 157.3 TFLOP/s",
                  "algorithmic_bytes_per_launch": by, "flops_per_launch": f64 + f32, "fp32_flops_per_launch": f32,
@@ -514,6 +520,27 @@ def fp64_64atom_leg():
                        "Schwarz bound >= 1e-12, plain FP64", "quartets": ss.total_quartets,
            "value": ss.total_quartets / wall, "unit": "shell quartets/s", "pass_s": wall,
            "tflops": ss.total_flops / wall / 1e12}
+    # the same pass with the Coulomb/exchange digestion after every launch
+    nbf = ss.nbf()
+    g = torch.Generator(device="cpu").manual_seed(5)
+    C = torch.rand((nbf, nbf // 4), generator=g, dtype=torch.float64) - 0.5
+    D = (C @ C.T / nbf).cuda()
+    J = torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda")
+    K = torch.zeros_like(J)
+    ss.run_jk(out.data_ptr(), D.data_ptr(), J.data_ptr(), K.data_ptr(), shares(ss.launches, 4)[0])
+    J.zero_()
+    K.zero_()
+    simint.synchronize()
+    t = time.perf_counter()
+    ss.run_jk(out.data_ptr(), D.data_ptr(), J.data_ptr(), K.data_ptr())
+    simint.jk_symmetrize(nbf, J.data_ptr(), K.data_ptr())
+    simint.synchronize()
+    wjk = time.perf_counter() - t
+    res["jk"] = {"what": "Coulomb and exchange matrices (%d basis functions) of a random density, digested after "
+                         "every class launch (simint_hip_compute_jk_range)" % nbf,
+                 "pass_s": wjk, "value": ss.total_quartets / wjk, "unit": "shell quartets/s",
+                 "digestion_share": 1.0 - wall / wjk,
+                 "energy_check": {"tr_DJ": float((D * J).sum()), "tr_DK": float((D * K).sum())}}
     ss.free()
     return res
 
@@ -537,10 +564,14 @@ def main():
     ap.add_argument("--copy-gib", type=float, default=None,
                     help="profiling aid: run one device copy of this many GiB and exit "
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--calib-flops", type=int, default=None,
+                    help="profiling aid: symmetrise one N x N FP64 matrix with the library's kernel "
+                         "(exactly N(N-1) FP64 flops: one add and one multiply per off-diagonal pair) "
+                         "and exit (calibrates the SQ_INSTS_VALU_FLOPS_FP64 counter)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(shell quartets are independent; sharding them over GPUs is described in DESIGN.md)",
+        print("bench.py measures one GPU: there is no multi-GPU path (shell quartets are independent; a "
+              "multi-GPU run would shard the screened set by bra pairs with replicated shell pairs)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -553,6 +584,17 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 8 * n}))
+        return
+    if args.calib_flops:
+        sys.path.insert(0, PKG)
+        import simint
+        simint.lib().simint_init()
+        n = args.calib_flops
+        M = torch.rand((n, n), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        simint.jk_symmetrize(n, M.data_ptr(), None)
+        simint.synchronize()
+        print(json.dumps({"calib_flops_fp64": n * (n - 1)}))
         return
     sys.path.insert(0, PKG)
     # the CPU oracle (error check and cpu_baseline only): oracle/oracle.py
@@ -567,9 +609,9 @@ def main():
     fp64_value = m["quartets"] / m["wall_fp64"]
     roof, agg = roofline_of(m["per_launch"], info)
     per_class = {}
-    for (cls, prec), a in agg.items():
-        pc = per_class.setdefault(cname(cls), {"ms_mixed": 0.0})
-        pc["ms_mixed"] += 1000.0 * a["s"]
+    for ln, s in m["per_launch"]:
+        pc = per_class.setdefault(cname(ln.cls), {"ms_mixed": 0.0})
+        pc["ms_mixed"] += 1000.0 * s
     for ln, s in m["per_launch64"]:
         pc = per_class.setdefault(cname(ln.cls), {"ms_mixed": 0.0})
         pc["ms_fp64"] = pc.get("ms_fp64", 0.0) + 1000.0 * s

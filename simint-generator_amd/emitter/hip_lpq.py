@@ -139,6 +139,17 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     def cname(k, kind):
         return ("one_over_%s" % kind) if k == 1 else ("c%d_%s" % (k, kind))
 
+    # each top is added to its accumulator right after it is computed, so
+    # the value is dead unless a later node reads it (the node list is a
+    # depth-first post-order over the tops)
+    acc_of = {}
+    for accname, key in acc_tops:
+        acc_of.setdefault(key, []).append(accname)
+
+    def accumulate(key, v):
+        for accname in acc_of.get(key, ()):
+            w(indent + "%s += %s;" % (accname, ("(double)%s" % v) if mixed else v))
+
     idx = {}
     for n in nodes:
         key = (n.q, n.m)
@@ -146,6 +157,7 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
         idx[key] = v
         if n.step is None:
             w(indent + "const %s %s = %s;" % (fp, v, cv("F[%d] * prefac" % n.m)))
+            accumulate(key, v)
             continue
         d = n.dir
         bra = n.step in (I, J)
@@ -173,11 +185,7 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
         if s[7] is not None:
             expr = "%s(%s, %s, %s)" % (fm, cname(n.fac[3], "2pq"), s[7], expr)
         w(indent + "const %s %s = %s;" % (fp, v, expr))
-    for accname, key in acc_tops:
-        if mixed:
-            w(indent + "%s += (double)%s;" % (accname, idx[key]))
-        else:
-            w(indent + "%s += %s;" % (accname, idx[key]))
+        accumulate(key, v)
 
 
 STAGE = 16   # output components per LDS slice (coalesced stores)
