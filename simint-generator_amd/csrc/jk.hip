@@ -17,6 +17,8 @@
 // each lane reads its quartet's integrals where the kernel wrote them.  The
 // bra-side J sums are reduced across the wave before one atomic add per
 // element; the ket-side J and the four exchange blocks are added per lane.
+// The exchange blocks whose row is a bra function (K_ac, K_ad) are summed
+// over b (and K_bc, K_bd over a) in registers before their atomic adds.
 #include <hip/hip_runtime.h>
 
 #include "simint/simint_eri.h"
@@ -125,6 +127,169 @@ __global__ void __launch_bounds__(256) jk_digest(const WorkItem *items, long lon
     }
 }
 
+// The same digestion with the ket shells' and the second bra shell's
+// component counts fixed at compile time: each lane reads every row
+// (a, b, :, :) of its quartet once and applies all six updates from
+// registers (the density values it needs are loaded once per item, or once
+// per a), instead of one pass over the quartet per update.
+template <int NB, int NC, int ND>
+__global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long long nitems, const long long *col,
+                                                   const double *ints, int na, JkDev jk)
+{
+    constexpr int NCD = NC * ND;
+    const long long w = (long long)blockIdx.x * 4 + threadIdx.x / 64;
+    if (w >= nitems) return;   // uniform over the wave
+    const int lane = threadIdx.x & 63;
+    const WorkItem it = items[w];
+    const int n = jk.nbf;
+    const int fa = jk.bra_fun[2 * it.bra], fb = jk.bra_fun[2 * it.bra + 1];
+    if (fa < 0 || fb < 0 || fa + na > n || fb + NB > n) return;
+    bool act = lane >= it.lane_lo && lane < it.lane_hi;
+    const long long k = act ? col[(long long)it.tile * 64 + lane] : -1;
+    act = act && k >= 0;
+    int fc = 0, fd = 0;
+    if (act) {
+        fc = jk.ket_fun[2 * k];
+        fd = jk.ket_fun[2 * k + 1];
+        act = fc >= 0 && fd >= 0 && fc + NC <= n && fd + ND <= n;
+    }
+    // idle lanes read a quartet of the item (its first lane's, which the
+    // class kernel wrote) and density element 0, and add nothing (weight 0)
+    const long long kq = act ? k : col[(long long)it.tile * 64 + it.lane_lo];
+    const double *q = ints + it.obase + kq * (long long)(na * NB * NCD);
+    const double *__restrict__ D = jk.D;
+    auto Dl = [&](long long idx) { return D[act ? idx : 0]; };
+    double f = act ? 1.0 : 0.0;
+    if (fa == fb) f *= 0.5;
+    if (fc == fd) f *= 0.5;
+    if (jk.same && k == it.bra) f *= 0.5;
+    double dcd[NCD], dbd[NB][ND], dbc[NB][NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) dcd[c * ND + d] = Dl((long long)(fc + c) * n + fd + d);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) dbd[b][d] = Dl((long long)(fb + b) * n + fd + d);
+#pragma unroll
+        for (int c = 0; c < NC; c++) dbc[b][c] = Dl((long long)(fb + b) * n + fc + c);
+    }
+    double jcd[NCD], kbc[NB][NC], kbd[NB][ND];
+#pragma unroll
+    for (int i = 0; i < NCD; i++) jcd[i] = 0.0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) kbc[b][c] = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; d++) kbd[b][d] = 0.0;
+    }
+    for (int a = 0; a < na; a++) {
+        double dad[ND], dac[NC], kac[NC], kad[ND];
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            dad[d] = Dl((long long)(fa + a) * n + fd + d);
+            kad[d] = 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            dac[c] = Dl((long long)(fa + a) * n + fc + c);
+            kac[c] = 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const double *r = q + (a * NB + b) * NCD;
+            const double dab = D[(long long)(fa + a) * n + fb + b];
+            double jab = 0.0;
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int d = 0; d < ND; d++) {
+                    const double v = act ? r[c * ND + d] : 0.0;   // r is readable either way
+                    jab = fma(v, dcd[c * ND + d], jab);
+                    jcd[c * ND + d] = fma(v, dab, jcd[c * ND + d]);
+                    kac[c] = fma(v, dbd[b][d], kac[c]);
+                    kad[d] = fma(v, dbc[b][c], kad[d]);
+                    kbc[b][c] = fma(v, dad[d], kbc[b][c]);
+                    kbd[b][d] = fma(v, dac[c], kbd[b][d]);
+                }
+            if (jk.J) {
+                jab = wave_sum(f * jab);
+                if (lane == 0) unsafeAtomicAdd(jk.J + (long long)(fa + a) * n + fb + b, 4.0 * jab);
+            }
+        }
+        if (jk.K && act) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) unsafeAtomicAdd(jk.K + (long long)(fa + a) * n + fc + c, 2.0 * f * kac[c]);
+#pragma unroll
+            for (int d = 0; d < ND; d++) unsafeAtomicAdd(jk.K + (long long)(fa + a) * n + fd + d, 2.0 * f * kad[d]);
+        }
+    }
+    if (!act) return;
+    if (jk.J) {
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int d = 0; d < ND; d++)
+                unsafeAtomicAdd(jk.J + (long long)(fc + c) * n + fd + d, 4.0 * f * jcd[c * ND + d]);
+    }
+    if (jk.K) {
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) unsafeAtomicAdd(jk.K + (long long)(fb + b) * n + fc + c, 2.0 * f * kbc[b][c]);
+#pragma unroll
+            for (int d = 0; d < ND; d++) unsafeAtomicAdd(jk.K + (long long)(fb + b) * n + fd + d, 2.0 * f * kbd[b][d]);
+        }
+    }
+}
+
+constexpr int nc_of(int l) { return (l + 1) * (l + 2) / 2; }
+
+// register arrays of the specialised kernel (doubles per lane); larger
+// shapes take the generic kernel
+constexpr bool fits_t(int nb, int nc, int nd) { return 3 * nc * nd + 2 * nb * (nc + nd) + 2 * (nc + nd) <= 192; }
+
+template <int LB, int LC, int LD>
+hipError_t launch_t(const WorkItem *items, long long nitems, const long long *col, const double *ints, int na,
+                    const JkDev &d, hipStream_t s)
+{
+    constexpr int NB = nc_of(LB), NC = nc_of(LC), ND = nc_of(LD);
+    if constexpr (fits_t(NB, NC, ND)) {
+        hipLaunchKernelGGL((jk_digest_t<NB, NC, ND>), dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, items,
+                           nitems, col, ints, na, d);
+    } else {
+        hipLaunchKernelGGL(jk_digest, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, items, nitems, col, ints,
+                           na, NB, NC, ND, d);
+    }
+    return hipGetLastError();
+}
+
+template <int LB, int LC>
+hipError_t launch_d(int ld, const WorkItem *items, long long nitems, const long long *col, const double *ints,
+                    int na, const JkDev &d, hipStream_t s)
+{
+    switch (ld) {
+    case 0: return launch_t<LB, LC, 0>(items, nitems, col, ints, na, d, s);
+    case 1: return launch_t<LB, LC, 1>(items, nitems, col, ints, na, d, s);
+    case 2: return launch_t<LB, LC, 2>(items, nitems, col, ints, na, d, s);
+    default: return launch_t<LB, LC, 3>(items, nitems, col, ints, na, d, s);
+    }
+}
+
+template <int LB>
+hipError_t launch_c(int lc, int ld, const WorkItem *items, long long nitems, const long long *col, const double *ints,
+                    int na, const JkDev &d, hipStream_t s)
+{
+    switch (lc) {
+    case 0: return launch_d<LB, 0>(ld, items, nitems, col, ints, na, d, s);
+    case 1: return launch_d<LB, 1>(ld, items, nitems, col, ints, na, d, s);
+    case 2: return launch_d<LB, 2>(ld, items, nitems, col, ints, na, d, s);
+    default: return launch_d<LB, 3>(ld, items, nitems, col, ints, na, d, s);
+    }
+}
+
 // M <- (M + M^T)/2: thread (i, j) with i < j averages the pair
 __global__ void __launch_bounds__(256) symmetrize(int n, double *M)
 {
@@ -147,10 +312,18 @@ hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long ni
                             const double *ints, const simint_hip_jk *jk, hipStream_t s)
 {
     JkDev d{jk->nbf, jk->same, jk->bra_fun, jk->ket_fun, jk->density, jk->J, jk->K};
-    const long long nblk = (nitems + 3) / 4;
-    hipLaunchKernelGGL(jk_digest, dim3((unsigned)nblk), dim3(256), 0, s, items, nitems, col, ints, ncart(am[0]),
-                       ncart(am[1]), ncart(am[2]), ncart(am[3]), d);
-    return hipGetLastError();
+    if (am[1] > 3 || am[2] > 3 || am[3] > 3) {
+        hipLaunchKernelGGL(jk_digest, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, items, nitems, col, ints,
+                           ncart(am[0]), ncart(am[1]), ncart(am[2]), ncart(am[3]), d);
+        return hipGetLastError();
+    }
+    const int na = ncart(am[0]);
+    switch (am[1]) {
+    case 0: return launch_c<0>(am[2], am[3], items, nitems, col, ints, na, d, s);
+    case 1: return launch_c<1>(am[2], am[3], items, nitems, col, ints, na, d, s);
+    case 2: return launch_c<2>(am[2], am[3], items, nitems, col, ints, na, d, s);
+    default: return launch_c<3>(am[2], am[3], items, nitems, col, ints, na, d, s);
+    }
 }
 
 }  // namespace simint_rt
