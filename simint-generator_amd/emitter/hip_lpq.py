@@ -39,14 +39,43 @@ class _Writer:
         return "\n".join(self.lines) + "\n"
 
 
-def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
+def emit_base(am, w, indent):
+    """The primitive quartet's base values, into variables declared by the
+    caller: Fp[m] = F_m(x) times the prefactor (m = 0..L), one_over_PQalpha_sum,
+    alpha, PQx/PQy/PQz -- the same arithmetic emit_body uses."""
+    Ltot = graph(*am).Ltot
+    w(indent + "{")
+    w(indent + "    const double PQalpha_mul = P_alpha * Q_alpha;")
+    w(indent + "    const double PQalpha_sum = P_alpha + Q_alpha;")
+    w(indent + "    one_over_PQalpha_sum = 1.0 / PQalpha_sum;")
+    w(indent + "    PQx = P_x - Q_x;")
+    w(indent + "    PQy = P_y - Q_y;")
+    w(indent + "    PQz = P_z - Q_z;")
+    w(indent + "    double R2 = PQx * PQx;")
+    w(indent + "    R2 = fma(PQy, PQy, R2);")
+    w(indent + "    R2 = fma(PQz, PQz, R2);")
+    w(indent + "    alpha = PQalpha_mul * one_over_PQalpha_sum;")
+    w(indent + "    const double F_x = R2 * alpha;")
+    w(indent + "    double F[%d];" % (Ltot + 1))
+    w(indent + "    boys_eval<%d>(grid, F_x, F);" % Ltot)
+    w(indent + "    const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
+    for m in range(Ltot + 1):
+        w(indent + "    Fp[%d] = F[%d] * prefac;" % (m, m))
+    w(indent + "}")
+
+
+def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None, base="compute"):
     """Emit the per-primitive-quartet code (setup, Boys, VRR, accumulation).
 
     Assumes in scope: P_alpha, P_inv, P_prefac, P_x/y/z, P_PA or P_PB (x,y,z),
     Q_* likewise, grid, and accumulators acc<k> (double).  fp = "float":
     the Gaussian-product setup, the Boys function and the prefactor stay
     FP64; the VRR runs in FP32 on rounded copies of its inputs, and each
-    result is accumulated into the FP64 accumulators."""
+    result is accumulated into the FP64 accumulators.  base = "given": the
+    values emit_base computes are already in scope (Fp[m], alpha,
+    one_over_PQalpha_sum, PQx/PQy/PQz), so only the factors derived from
+    them and the VRR are emitted."""
+    given = base == "given"
     mixed = fp == "float"
     fm = "fmaf" if mixed else "fma"
     g = graph(*am)
@@ -63,13 +92,17 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     # transfers between tops of different groups cancel -- with per-group
     # orders they did not, and (df|dd) lost 1e-7 normwise.
     used_steps = set(n.step for n in nodes if n.step is not None)
-    has_bra = bool(used_steps & {I, J})
-    has_ket = bool(used_steps & {K, L})
     # classes without HRR take 1/(p+q) and its square root from one refined
     # reciprocal square root (a few ulp): no HRR transfer amplifies it.  Only
     # in the lane-per-quartet kernels (their file carries the helpers); the
     # setup is FP64 in the mixed kernels too.
-    fast = keep is None and not g.bra_hrr_needed() and not g.ket_hrr_needed()
+    fast = keep is None and not g.bra_hrr_needed() and not g.ket_hrr_needed() and not given
+    if not given:
+        _emit_setup(w, indent, Ltot, fast)
+    _emit_vrr(g, nodes, acc_tops, w, indent, fp, mixed, fm, given, used_steps)
+
+
+def _emit_setup(w, indent, Ltot, fast):
     w(indent + "const double PQalpha_mul = P_alpha * Q_alpha;")
     w(indent + "const double PQalpha_sum = P_alpha + Q_alpha;")
     if fast:
@@ -92,6 +125,11 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
     else:
         w(indent + "boys_eval<%d>(grid, F_x, F);" % Ltot)
         w(indent + "const double prefac = (P_prefac * Q_prefac) * sqrt(one_over_PQalpha_sum);")
+
+
+def _emit_vrr(g, nodes, acc_tops, w, indent, fp, mixed, fm, given, used_steps):
+    has_bra = bool(used_steps & {I, J})
+    has_ket = bool(used_steps & {K, L})
     # integer constants k/(2p), k/(2q), k/(2pq)
     need = {"2p": set(), "2q": set(), "2pq": set()}
     for n in nodes:
@@ -156,7 +194,7 @@ def emit_body(am, w, indent="        ", fp="double", keep=None, acc_tops=None):
         v = "v%d" % len(idx)
         idx[key] = v
         if n.step is None:
-            w(indent + "const %s %s = %s;" % (fp, v, cv("F[%d] * prefac" % n.m)))
+            w(indent + "const %s %s = %s;" % (fp, v, cv(("Fp[%d]" if given else "F[%d] * prefac") % n.m)))
             accumulate(key, v)
             continue
         d = n.dir
