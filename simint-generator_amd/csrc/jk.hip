@@ -131,16 +131,16 @@ __global__ void __launch_bounds__(256) jk_digest(const WorkItem *items, long lon
 // component counts fixed at compile time: each lane reads every row
 // (a, b, :, :) of its quartet once and applies all six updates from
 // registers (the density values it needs are loaded once per item, or once
-// per a), instead of one pass over the quartet per update.
-template <int NB, int NC, int ND>
-__global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long long nitems, const long long *col,
-                                                   const double *ints, int na, JkDev jk)
+// per a), instead of one pass over the quartet per update.  rows != nullptr:
+// the exchange updates whose row is a bra function (K_a., K_b.) go to that
+// block's row accumulator in the local data share -- row a of the first bra
+// shell at rows[a * nbf], row b of the second at rows[(na + b) * nbf] --
+// instead of global memory.
+template <int NB, int NC, int ND, bool ROWS>
+__device__ __forceinline__ void digest_item(const WorkItem &it, int lane, const long long *col, const double *ints,
+                                            int na, const JkDev &jk, double *rows)
 {
     constexpr int NCD = NC * ND;
-    const long long w = (long long)blockIdx.x * 4 + threadIdx.x / 64;
-    if (w >= nitems) return;   // uniform over the wave
-    const int lane = threadIdx.x & 63;
-    const WorkItem it = items[w];
     const int n = jk.nbf;
     const int fa = jk.bra_fun[2 * it.bra], fb = jk.bra_fun[2 * it.bra + 1];
     if (fa < 0 || fb < 0 || fa + na > n || fb + NB > n) return;
@@ -159,6 +159,16 @@ __global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long l
     const double *q = ints + it.obase + kq * (long long)(na * NB * NCD);
     const double *__restrict__ D = jk.D;
     auto Dl = [&](long long idx) { return D[act ? idx : 0]; };
+    // exchange row update: row r of the bra's (na + NB) rows, column c
+    auto addK = [&](int r, int c, double v) {
+        if constexpr (ROWS) {
+            // the local data share, named as such so the add is a ds_add_f64
+            __attribute__((address_space(3))) double *lds = (__attribute__((address_space(3))) double *)rows;
+            atomicAdd((double *)(lds + (long long)r * n + c), v);
+        } else {
+            unsafeAtomicAdd(jk.K + (long long)(r < na ? fa + r : fb + r - na) * n + c, v);
+        }
+    };
     double f = act ? 1.0 : 0.0;
     if (fa == fb) f *= 0.5;
     if (fc == fd) f *= 0.5;
@@ -221,9 +231,9 @@ __global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long l
         }
         if (jk.K && act) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) unsafeAtomicAdd(jk.K + (long long)(fa + a) * n + fc + c, 2.0 * f * kac[c]);
+            for (int c = 0; c < NC; c++) addK(a, fc + c, 2.0 * f * kac[c]);
 #pragma unroll
-            for (int d = 0; d < ND; d++) unsafeAtomicAdd(jk.K + (long long)(fa + a) * n + fd + d, 2.0 * f * kad[d]);
+            for (int d = 0; d < ND; d++) addK(a, fd + d, 2.0 * f * kad[d]);
         }
     }
     if (!act) return;
@@ -238,10 +248,62 @@ __global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long l
 #pragma unroll
         for (int b = 0; b < NB; b++) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) unsafeAtomicAdd(jk.K + (long long)(fb + b) * n + fc + c, 2.0 * f * kbc[b][c]);
+            for (int c = 0; c < NC; c++) addK(na + b, fc + c, 2.0 * f * kbc[b][c]);
 #pragma unroll
-            for (int d = 0; d < ND; d++) unsafeAtomicAdd(jk.K + (long long)(fb + b) * n + fd + d, 2.0 * f * kbd[b][d]);
+            for (int d = 0; d < ND; d++) addK(na + b, fd + d, 2.0 * f * kbd[b][d]);
         }
+    }
+}
+
+// one wavefront per work item
+template <int NB, int NC, int ND>
+__global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long long nitems, const long long *col,
+                                                   const double *ints, int na, JkDev jk)
+{
+    const long long w = (long long)blockIdx.x * 4 + threadIdx.x / 64;
+    if (w >= nitems) return;   // uniform over the wave
+    digest_item<NB, NC, ND, false>(items[w], threadIdx.x & 63, col, ints, na, jk, nullptr);
+}
+
+// One block per bra pair of the launch: its four wavefronts take the bra's
+// work items in turn, and the exchange rows of the bra's functions are
+// summed over all its kets in the local data share ((na + NB) x nbf
+// doubles, dynamic) and added to K once per column at the end -- when the
+// bra meets enough kets that this is fewer global atomics than one per
+// quartet and row.
+template <int NB, int NC, int ND>
+__global__ void __launch_bounds__(256) jk_digest_rows(const WorkItem *items, long long nitems,
+                                                      const long long *tstart, int nbra, const long long *col,
+                                                      const double *ints, int na, JkDev jk)
+{
+    extern __shared__ double rows[];
+    const int bi = blockIdx.x;
+    const long long i0 = tstart[bi], i1 = bi + 1 < nbra ? tstart[bi + 1] : nitems;
+    if (i0 >= i1) return;   // uniform over the block
+    const int n = jk.nbf;
+    const int R = na + NB;
+    const bool use_rows = jk.K && (i1 - i0) * 64 * (NC + ND) > 2LL * R * n;
+    if (use_rows) {
+        for (long long e = threadIdx.x; e < (long long)R * n; e += 256) rows[e] = 0.0;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63;
+    if (!use_rows) {
+        for (long long w = i0 + threadIdx.x / 64; w < i1; w += 4)
+            digest_item<NB, NC, ND, false>(items[w], lane, col, ints, na, jk, nullptr);
+        return;
+    }
+    for (long long w = i0 + threadIdx.x / 64; w < i1; w += 4)
+        digest_item<NB, NC, ND, true>(items[w], lane, col, ints, na, jk, rows);
+    __syncthreads();
+    const int bra = items[i0].bra;
+    const int fa = jk.bra_fun[2 * bra], fb = jk.bra_fun[2 * bra + 1];
+    if (fa < 0 || fb < 0 || fa + na > n || fb + NB > n) return;
+    for (long long e = threadIdx.x; e < (long long)R * n; e += 256) {
+        const double v = rows[e];
+        if (v == 0.0) continue;
+        const int r = (int)(e / n), c = (int)(e - (long long)r * n);
+        unsafeAtomicAdd(jk.K + (long long)(r < na ? fa + r : fb + r - na) * n + c, v);
     }
 }
 
@@ -251,15 +313,51 @@ constexpr int nc_of(int l) { return (l + 1) * (l + 2) / 2; }
 // shapes take the generic kernel
 constexpr bool fits_t(int nb, int nc, int nd) { return 3 * nc * nd + 2 * nb * (nc + nd) + 2 * (nc + nd) <= 192; }
 
+// exchange row accumulators of the per-bra kernel: at most this many bytes
+// of the local data share per block (and no more than the device allows)
+constexpr long long kRowBytes = 96 * 1024;
+
+long long row_bytes_limit()
+{
+    static long long lim = -1;
+    if (lim < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        lim = v < kRowBytes ? v : kRowBytes;
+    }
+    return lim;
+}
+
+struct Launch {
+    const WorkItem *items;
+    long long nitems;
+    const long long *tstart;
+    int nbra;
+    const long long *col;
+    const double *ints;
+    int na;
+};
+
 template <int LB, int LC, int LD>
-hipError_t launch_t(const WorkItem *items, long long nitems, const long long *col, const double *ints, int na,
-                    const JkDev &d, hipStream_t s)
+hipError_t launch_t(const Launch &L, const JkDev &d, hipStream_t s)
 {
     constexpr int NB = nc_of(LB), NC = nc_of(LC), ND = nc_of(LD);
+    const long long rbytes = 8LL * (L.na + NB) * d.nbf;
     if constexpr (fits_t(NB, NC, ND)) {
-        hipLaunchKernelGGL((jk_digest_t<NB, NC, ND>), dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, items,
-                           nitems, col, ints, na, d);
+        if (d.K && rbytes <= row_bytes_limit())
+            hipLaunchKernelGGL((jk_digest_rows<NB, NC, ND>), dim3((unsigned)L.nbra), dim3(256), (size_t)rbytes, s,
+                               L.items, L.nitems, L.tstart, L.nbra, L.col, L.ints, L.na, d);
+        else
+            hipLaunchKernelGGL((jk_digest_t<NB, NC, ND>), dim3((unsigned)((L.nitems + 3) / 4)), dim3(256), 0, s,
+                               L.items, L.nitems, L.col, L.ints, L.na, d);
     } else {
+        const WorkItem *items = L.items;
+        const long long nitems = L.nitems;
+        const long long *col = L.col;
+        const double *ints = L.ints;
+        const int na = L.na;
         hipLaunchKernelGGL(jk_digest, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, items, nitems, col, ints,
                            na, NB, NC, ND, d);
     }
@@ -267,26 +365,24 @@ hipError_t launch_t(const WorkItem *items, long long nitems, const long long *co
 }
 
 template <int LB, int LC>
-hipError_t launch_d(int ld, const WorkItem *items, long long nitems, const long long *col, const double *ints,
-                    int na, const JkDev &d, hipStream_t s)
+hipError_t launch_d(int ld, const Launch &L, const JkDev &d, hipStream_t s)
 {
     switch (ld) {
-    case 0: return launch_t<LB, LC, 0>(items, nitems, col, ints, na, d, s);
-    case 1: return launch_t<LB, LC, 1>(items, nitems, col, ints, na, d, s);
-    case 2: return launch_t<LB, LC, 2>(items, nitems, col, ints, na, d, s);
-    default: return launch_t<LB, LC, 3>(items, nitems, col, ints, na, d, s);
+    case 0: return launch_t<LB, LC, 0>(L, d, s);
+    case 1: return launch_t<LB, LC, 1>(L, d, s);
+    case 2: return launch_t<LB, LC, 2>(L, d, s);
+    default: return launch_t<LB, LC, 3>(L, d, s);
     }
 }
 
 template <int LB>
-hipError_t launch_c(int lc, int ld, const WorkItem *items, long long nitems, const long long *col, const double *ints,
-                    int na, const JkDev &d, hipStream_t s)
+hipError_t launch_c(int lc, int ld, const Launch &L, const JkDev &d, hipStream_t s)
 {
     switch (lc) {
-    case 0: return launch_d<LB, 0>(ld, items, nitems, col, ints, na, d, s);
-    case 1: return launch_d<LB, 1>(ld, items, nitems, col, ints, na, d, s);
-    case 2: return launch_d<LB, 2>(ld, items, nitems, col, ints, na, d, s);
-    default: return launch_d<LB, 3>(ld, items, nitems, col, ints, na, d, s);
+    case 0: return launch_d<LB, 0>(ld, L, d, s);
+    case 1: return launch_d<LB, 1>(ld, L, d, s);
+    case 2: return launch_d<LB, 2>(ld, L, d, s);
+    default: return launch_d<LB, 3>(ld, L, d, s);
     }
 }
 
@@ -308,8 +404,9 @@ inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 
 namespace simint_rt {
 
-hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long nitems, const long long *col,
-                            const double *ints, const simint_hip_jk *jk, hipStream_t s)
+hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long nitems, const long long *tstart,
+                            int nbra, const long long *col, const double *ints, const simint_hip_jk *jk,
+                            hipStream_t s)
 {
     JkDev d{jk->nbf, jk->same, jk->bra_fun, jk->ket_fun, jk->density, jk->J, jk->K};
     if (am[1] > 3 || am[2] > 3 || am[3] > 3) {
@@ -317,12 +414,12 @@ hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long ni
                            ncart(am[0]), ncart(am[1]), ncart(am[2]), ncart(am[3]), d);
         return hipGetLastError();
     }
-    const int na = ncart(am[0]);
+    const Launch L{items, nitems, tstart, nbra, col, ints, ncart(am[0])};
     switch (am[1]) {
-    case 0: return launch_c<0>(am[2], am[3], items, nitems, col, ints, na, d, s);
-    case 1: return launch_c<1>(am[2], am[3], items, nitems, col, ints, na, d, s);
-    case 2: return launch_c<2>(am[2], am[3], items, nitems, col, ints, na, d, s);
-    default: return launch_c<3>(am[2], am[3], items, nitems, col, ints, na, d, s);
+    case 0: return launch_c<0>(am[2], am[3], L, d, s);
+    case 1: return launch_c<1>(am[2], am[3], L, d, s);
+    case 2: return launch_c<2>(am[2], am[3], L, d, s);
+    default: return launch_c<3>(am[2], am[3], L, d, s);
     }
 }
 

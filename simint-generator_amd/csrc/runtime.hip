@@ -942,7 +942,7 @@ struct RangeTerm {
 long long range_launch(const simint_hip_pairs *P, const simint_hip_pairs *Q, int b0, int nbra, const int *ket_lo,
                        const int *ket_hi, const long long *const *out_offs, const RangeTerm *terms, int nterm,
                        double screen_tol, double *const *d_outs, const WorkItem **items_out = nullptr,
-                       long long *nitems_out = nullptr)
+                       long long *nitems_out = nullptr, const long long **tstart_out = nullptr)
 {
     if (nitems_out) *nitems_out = 0;
     if (nbra == 0) return 0;
@@ -1020,6 +1020,7 @@ long long range_launch(const simint_hip_pairs *P, const simint_hip_pairs *Q, int
     }
     if (items_out) *items_out = static_cast<const WorkItem *>(c.items.p);
     if (nitems_out) *nitems_out = nitems;
+    if (tstart_out) *tstart_out = reinterpret_cast<const long long *>(base + o_ts);
     return nq;
 }
 
@@ -1054,11 +1055,13 @@ long long simint_hip_compute_jk_range(const simint_hip_pairs *P, const simint_hi
     if (!t.fn) t.fn = simint_dispatch(P->am1, P->am2, Q->am1, Q->am2);
     if (!t.fn) { t_err = "class not built"; return SIMINT_ERROR_UNSUPPORTED; }
     const WorkItem *items = nullptr;
+    const long long *tstart = nullptr;
     long long nitems = 0;
     const long long nq = range_launch(P, Q, b0, nbra, ket_lo, ket_hi, &out_off, &t, 1, screen_tol, &d_out, &items,
-                                      &nitems);
+                                      &nitems, &tstart);
     if (nq <= 0 || nitems == 0 || (!jk->J && !jk->K)) return nq;
-    const hipError_t e = simint_rt::launch_jk_digest(t.am, items, nitems, Q->til.col, d_out, jk, ctx().s);
+    const hipError_t e = simint_rt::launch_jk_digest(t.am, items, nitems, tstart, nbra, Q->til.col, d_out, jk,
+                                                     ctx().s);
     if (e != hipSuccess) { set_err("digest launch", e); return SIMINT_ERROR_HIP; }
     return nq;
 }

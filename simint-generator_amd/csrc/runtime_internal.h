@@ -64,8 +64,10 @@ void set_hip_error(const char *what, hipError_t e);
 // again in shell_screen.cpp, which is compiled as plain C++.
 int host_pair_bounds(const simint_multi_shellpair *P, double *out);
 // jk.hip: queue the J/K digestion of the integrals the class kernel of class
-// am wrote for `items` (ket tile lanes mapped to pairs by col) on stream s
-hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long nitems, const long long *col,
-                            const double *ints, const simint_hip_jk *jk, hipStream_t s);
+// am wrote for `items` (ket tile lanes mapped to pairs by col; the items of
+// bra i of the launch start at tstart[i], device, nbra entries) on stream s
+hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long nitems, const long long *tstart,
+                            int nbra, const long long *col, const double *ints, const simint_hip_jk *jk,
+                            hipStream_t s);
 
 }  // namespace simint_rt
