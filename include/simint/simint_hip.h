@@ -188,6 +188,11 @@ long long simint_hip_compute_jk_range(simint_hip_pairs const * P, simint_hip_pai
 /* J <- (J + J^T) / 2 and K <- (K + K^T) / 2 (device, nbf x nbf, either may
  * be NULL), on the calling thread's stream. */
 int simint_hip_jk_symmetrize(int nbf, double * d_J, double * d_K);
+/* Exchange updates of a bra pair's rows are summed in the local data share
+ * over all the bra's kets before they reach K (1, the default, where the
+ * rows fit) or added to K per quartet (0).  Process-wide; returns the
+ * previous setting. */
+int simint_hip_set_jk_rows(int enable);
 
 /* Cartesian order of the drop-in calls' output (simint_compute_eri,
  * simint_compute_eri_deriv and the one-electron calls), process-wide.

@@ -21,6 +21,8 @@
 // over b (and K_bc, K_bd over a) in registers before their atomic adds.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "simint/simint_eri.h"
 #include "simint/simint_hip.h"
 #include "device_types.h"
@@ -268,9 +270,9 @@ __global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long l
 // One block per bra pair of the launch: its four wavefronts take the bra's
 // work items in turn, and the exchange rows of the bra's functions are
 // summed over all its kets in the local data share ((na + NB) x nbf
-// doubles, dynamic) and added to K once per column at the end -- when the
-// bra meets enough kets that this is fewer global atomics than one per
-// quartet and row.
+// doubles, dynamic) and added to K once per touched element at the end --
+// never more global atomics than one per quartet and element, usually far
+// fewer (every ket of the bra adds to the same rows).
 template <int NB, int NC, int ND>
 __global__ void __launch_bounds__(256) jk_digest_rows(const WorkItem *items, long long nitems,
                                                       const long long *tstart, int nbra, const long long *col,
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__(256) jk_digest_rows(const WorkItem *items, lon
     if (i0 >= i1) return;   // uniform over the block
     const int n = jk.nbf;
     const int R = na + NB;
-    const bool use_rows = jk.K && (i1 - i0) * 64 * (NC + ND) > 2LL * R * n;
+    const bool use_rows = jk.K != nullptr;
     if (use_rows) {
         for (long long e = threadIdx.x; e < (long long)R * n; e += 256) rows[e] = 0.0;
         __syncthreads();
@@ -317,17 +319,21 @@ constexpr bool fits_t(int nb, int nc, int nd) { return 3 * nc * nd + 2 * nb * (n
 // of the local data share per block (and no more than the device allows)
 constexpr long long kRowBytes = 96 * 1024;
 
+std::atomic<int> g_rows_enabled{1};   // simint_hip_set_jk_rows
+
 long long row_bytes_limit()
 {
-    static long long lim = -1;
-    if (lim < 0) {
-        int dev = 0, v = 0;
+    static std::atomic<long long> lim{-1};
+    long long v = lim.load(std::memory_order_relaxed);
+    if (v < 0) {
+        int dev = 0, a = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            v = 65536;
-        lim = v < kRowBytes ? v : kRowBytes;
+            hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            a = 65536;
+        v = a < kRowBytes ? a : kRowBytes;
+        lim.store(v, std::memory_order_relaxed);
     }
-    return lim;
+    return g_rows_enabled.load(std::memory_order_relaxed) ? v : 0;
 }
 
 struct Launch {
@@ -426,6 +432,11 @@ hipError_t launch_jk_digest(const int am[4], const WorkItem *items, long long ni
 }  // namespace simint_rt
 
 extern "C" {
+
+int simint_hip_set_jk_rows(int enable)
+{
+    return g_rows_enabled.exchange(enable ? 1 : 0);
+}
 
 int simint_hip_jk_symmetrize(int nbf, double *d_J, double *d_K)
 {

@@ -19,10 +19,14 @@ The server
     no longer holds a slot for its whole life, and small ones that arrive
     behind it finish without waiting for it;
   * reduces every tile on the device into that request's per-tile digest
-    (sum of values and sum of squares in FP64, the stand-in for a consumer
-    such as a Fock build) and sums the per-tile digests in tile order at the
-    end, so a request's result does not depend on how its tiles were
-    scheduled -- it is the same alone and in the mix;
+    (sum of values and sum of squares in FP64) and sums the per-tile digests
+    in tile order at the end, so a request's result does not depend on how
+    its tiles were scheduled -- it is the same alone and in the mix;
+  * for a "jk" request (a Fock build's two-electron part), also digests
+    every tile's integrals into the request's Coulomb and exchange matrices
+    of a density (simint_hip_compute_jk_range; the density is the request's
+    own or one drawn from its seed) and returns J and K -- equal alone and
+    in the mix to the last bits, the atomic sums' order aside;
   * stops scheduling a request's tiles once it has run past its time limit
     and reports it as timed out.
 
@@ -54,10 +58,13 @@ MAXAM = 2   # shells the screened sets take (the recipe's s, p, d)
 
 
 class Request:
-    def __init__(self, rid, natom, seed, kind, arrival, shells=None, timeout=None):
+    def __init__(self, rid, natom, seed, kind, arrival, shells=None, timeout=None, density=None):
         self.rid, self.natom, self.seed, self.kind, self.arrival = rid, natom, seed, kind, arrival
         self.shells = shells       # explicit shells, or None: the recipe for (natom, seed)
         self.timeout = timeout     # seconds after admission, or None
+        self.density = density     # "jk": nbf x nbf symmetric density, or None: drawn from the seed
+        self.J = self.K = None     # "jk": the results (host arrays)
+        self.jk_dev = None
         self.t_admit = self.t_done = None
         self.quartets = 0
         self.digest = None
@@ -81,9 +88,11 @@ class Request:
                 "quartets": self.quartets, "digest": self.digest, "error": self.error}
 
 
-def request_stream(n, rate, seed=0, natom_range=(10, 200), deriv_frac=0.3):
+def request_stream(n, rate, seed=0, natom_range=(10, 200), deriv_frac=0.3, jk_frac=0.0):
     """n requests: Poisson arrivals at `rate` per second, atom counts uniform
-    in natom_range, 30% first-derivative requests, per-request seeds."""
+    in natom_range, 30% first-derivative requests, per-request seeds; with
+    jk_frac, that share of the ERI requests asks for J and K instead of the
+    digest alone."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -91,8 +100,18 @@ def request_stream(n, rate, seed=0, natom_range=(10, 200), deriv_frac=0.3):
         t += rng.exponential(1.0 / rate)
         natom = int(rng.integers(natom_range[0], natom_range[1] + 1))
         kind = "deriv" if rng.random() < deriv_frac else "eri"
+        if kind == "eri" and jk_frac and rng.random() < jk_frac:
+            kind = "jk"
         out.append(Request(i, natom, int(rng.integers(1, 2 ** 31)), kind, t))
     return out
+
+
+def request_density(nbf, seed):
+    """The density a "jk" request without its own uses: C C^T / nbf with C
+    uniform in [-1/2, 1/2), nbf x nbf/4 (positive semidefinite, symmetric)."""
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(-0.5, 0.5, (nbf, max(2, nbf // 4)))
+    return np.ascontiguousarray(C @ C.T / nbf)
 
 
 def validate(shells):
@@ -116,16 +135,19 @@ def validate(shells):
 def footprint(nshell, kind, chunk_doubles):
     """Estimated device bytes a request will hold before its set-up has run:
     its shell pairs (linear and tiled views, ~20 doubles per primitive pair,
-    6.25 primitive pairs per pair on average).  The output chunk buffers and
+    6.25 primitive pairs per pair on average) and, for "jk", the density, J
+    and K (at most 6 functions per shell, d).  The output chunk buffers and
     the derivative scratch belong to the workers, not to requests."""
     npair = nshell * (nshell + 1) // 2
-    return npair * 6.25 * 20 * 8 * 2
+    mats = 3 * 8 * (6 * nshell) ** 2 if kind == "jk" else 0
+    return npair * 6.25 * 20 * 8 * 2 + mats
 
 
 class Server:
     def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False, probe=0,
                  timeout=None):
         self.verbose = verbose
+        self.last_note = 0.0
         self.probe = probe         # quartets per request whose values are kept (result checks)
         self.budget = float(hbm_budget)
         self.free = float(hbm_budget)
@@ -175,21 +197,25 @@ class Server:
                 r.error = "rejected: footprint above the HBM budget"
                 r.t_admit = r.t_done = now
                 self._complete(r, self.done)
+        # admission comes first (one set-up at a time): a request that fits
+        # the remaining budget is set up as soon as it arrives, and its
+        # tiles then compete with the running requests' by remaining work
+        if self.setting_up == 0 or not live:
+            for r in self.waiting:
+                if footprint(self._nshell(r), r.kind, self.chunk) <= self.free:
+                    self.waiting.remove(r)
+                    est = footprint(self._nshell(r), r.kind, self.chunk)
+                    self.free -= est
+                    r.charged = est
+                    r.t_admit = now
+                    self.setting_up += 1
+                    return ("setup", r)
         if live:
             r = min(live, key=lambda x: (x.left, x.arrival))
             c = r.next
             r.next += 1
             r.inflight += 1
             return ("tile", r, c)
-        for r in self.waiting:
-            if footprint(self._nshell(r), r.kind, self.chunk) <= self.free:
-                self.waiting.remove(r)
-                est = footprint(self._nshell(r), r.kind, self.chunk)
-                self.free -= est
-                r.charged = est
-                r.t_admit = now
-                self.setting_up += 1
-                return ("setup", r)
         return None
 
     def _finish(self, r):
@@ -212,6 +238,16 @@ class Server:
         r.left = float(ss.total_bytes)
         r.sums = torch.zeros((max(ss.nchunk, 1), 2), dtype=torch.float64, device="cuda")
         r.work_need = ss.deriv_worksize() if deriv else 0
+        if r.kind == "jk":
+            nbf = ss.nbf()
+            D = r.density if r.density is not None else request_density(nbf, r.seed)
+            D = np.ascontiguousarray(D, dtype=np.float64)
+            if D.shape != (nbf, nbf) or not np.all(np.isfinite(D)) or np.abs(D - D.T).max() > 1e-12 * max(
+                    1.0, np.abs(D).max()):
+                r.error = "rejected: density must be a finite symmetric %d x %d matrix" % (nbf, nbf)
+                return
+            r.jk_dev = (torch.from_numpy(D).cuda(), torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda"),
+                        torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda"))
         torch.cuda.synchronize()
         if self.probe:
             for c, off, nc, q in ss.located_sample(self.probe, seed=r.seed):
@@ -230,7 +266,11 @@ class Server:
                 work[0] = torch.empty(need, dtype=torch.float64, device="cuda")
             wp = work[0].data_ptr()
         with torch.cuda.stream(stream):
-            ss.run_pass(out.data_ptr(), chunk=c, work_ptr=wp)
+            if r.jk_dev is not None:
+                D, J, K = r.jk_dev
+                ss.run_jk(out.data_ptr(), D.data_ptr(), J.data_ptr(), K.data_ptr(), ss.chunk_launches(c))
+            else:
+                ss.run_pass(out.data_ptr(), chunk=c, work_ptr=wp)
             x = out[:n]
             r.sums[c, 0] = x.sum()
             r.sums[c, 1] = torch.dot(x, x)
@@ -262,6 +302,8 @@ class Server:
                     self.setting_up -= 1
                     if r.ss is not None:
                         actual = float(r.ss.device_bytes()) + r.sums.numel() * 8.0
+                        if r.jk_dev is not None:
+                            actual += sum(m.numel() * 8.0 for m in r.jk_dev)
                         self.free += r.charged - actual
                         r.charged = actual
                     if r.error is None and r.ss is not None and r.ss.nchunk > 0:
@@ -284,6 +326,11 @@ class Server:
                 self.tiles_run += 1
                 r.inflight -= 1
                 r.done_tiles += 1
+                if self.verbose and self.now() - self.last_note > 30.0:
+                    # a progress line while long requests run
+                    self.last_note = self.now()
+                    print("[serve %.1fs] %d requests running; request %d: %d of %d tiles"
+                          % (self.now(), len(self.active), r.rid, r.done_tiles, r.ss.nchunk), flush=True)
                 r.left -= nbytes
                 r.kept.extend(kept)
                 if err is not None and r.error is None:
@@ -299,6 +346,14 @@ class Server:
             r.digest = [float(v) for v in r.sums.sum(dim=0).cpu()]
             if self.probe:
                 r.samples = sorted(r.kept, key=lambda qv: qv[0])
+            if r.jk_dev is not None:
+                # every tile's stream was synchronised when the tile ended
+                D, J, K = r.jk_dev
+                simint.jk_symmetrize(J.shape[0], J.data_ptr(), K.data_ptr())
+                simint.synchronize()
+                r.J, r.K = J.cpu().numpy(), K.cpu().numpy()
+                r.digest += [float((D * J).sum()), float((D * K).sum())]
+        r.jk_dev = None
         if r.ss is not None:
             r.ss.free()
             r.ss = None
@@ -360,8 +415,10 @@ def main():
     ap.add_argument("--max-atoms", type=int, default=200)
     ap.add_argument("--budget-gib", type=float, default=64.0)
     ap.add_argument("--timeout", type=float, default=None, help="per-request limit, seconds after admission")
+    ap.add_argument("--jk-frac", type=float, default=0.0,
+                    help="share of the ERI requests that ask for Coulomb and exchange matrices")
     a = ap.parse_args()
-    reqs = request_stream(a.requests, a.rate, a.seed, (a.min_atoms, a.max_atoms))
+    reqs = request_stream(a.requests, a.rate, a.seed, (a.min_atoms, a.max_atoms), jk_frac=a.jk_frac)
     srv = Server(hbm_budget=a.budget_gib * (1 << 30), slots=a.slots, tol=a.tol, verbose=True, timeout=a.timeout)
     srv.serve(reqs)
     res = summarize(reqs, srv.wall)

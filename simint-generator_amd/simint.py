@@ -67,7 +67,7 @@ EXPORTS = [
     "simint_hip_compute_eri_deriv1_list_noalloc", "simint_hip_pairs_bytes",
     "simint_hip_set_ordering", "simint_hip_ordering",
     "simint_hip_pairs_from_shells", "simint_hip_pair_bounds", "simint_hip_order_pairs", "simint_hip_screen_counts",
-    "simint_hip_pairs_export", "simint_hip_compute_jk_range", "simint_hip_jk_symmetrize",
+    "simint_hip_pairs_export", "simint_hip_compute_jk_range", "simint_hip_jk_symmetrize", "simint_hip_set_jk_rows",
 ]
 ORDERING_PSI4, ORDERING_GAMESS = 0, 1
 

@@ -116,3 +116,33 @@ def test_jk_only_one_of_them(gpu):
     _, K1, _ = _gpu_jk(shells, D, want=("K",))
     assert np.abs(J - J1).max() <= TOL * np.abs(J).max()
     assert np.abs(K - K1).max() <= TOL * np.abs(K).max()
+
+
+@pytest.mark.parametrize("rows", [0, 1])
+def test_jk_exchange_paths(gpu, rows):
+    """The exchange updates summed per bra in the local data share (rows 1,
+    the default) and added to K quartet by quartet (rows 0) both give the
+    oracle's K, on shells up to f with tight exponents."""
+    L = simint.lib()
+    prev = L.simint_hip_set_jk_rows(rows)
+    try:
+        assert _compare(_f_set(seed=9) + workloads.config2(natom=1, seed=9), 5)
+    finally:
+        L.simint_hip_set_jk_rows(prev)
+
+
+def test_jk_mixed_plan(gpu):
+    """A mixed plan (quartets with small Schwarz bounds on the FP32-VRR
+    kernels) digested the same way, on two atoms of the 64-atom recipe: J
+    and K within 1e-9 absolute of the oracle's -- each FP32 integral is
+    within the mixed path's 1e-10, and a J or K element sums such errors
+    weighted by density elements of at most ~0.1."""
+    shells = workloads.config2(natom=2, seed=11)
+    sa = normalized_arrays(shells)
+    g, _ = oracle.eri_tensor(sa)
+    D = _density(g.shape[0], 6)
+    Jr, Kr = oracle.jk(g, D)
+    J, K, _ = _gpu_jk(shells, D, mixed=1e-4)
+    ej, ek = np.abs(J - Jr).max(), np.abs(K - Kr).max()
+    print("mixed plan: max |dJ| %.2e, max |dK| %.2e" % (ej, ek))
+    assert ej <= 1e-9 and ek <= 1e-9

@@ -120,8 +120,10 @@ def test_small_request_overtakes_large_one(gpu):
     srv.serve([big, small])
     assert big.error is None and small.error is None
     print("big", big.t_done - big.arrival, "small", small.t_done - small.arrival, "tiles", srv.tiles_run)
-    # the small one arrived while the big one was still being served
+    # the small one arrived while the big one was still being served, was
+    # admitted at once and finished first
     assert small.arrival < big.t_done
+    assert small.t_admit - small.arrival < 1.0, (small.t_admit, small.arrival)
     assert small.t_done < big.t_done
 
 
@@ -160,3 +162,50 @@ def test_screened_deriv_pass_vs_oracle(gpu):
     print(st)
     assert ok, st
     ss.free()
+
+
+def test_jk_requests_vs_oracle_alone_and_in_mix(gpu):
+    """"jk" requests (Coulomb and exchange matrices of a density, the
+    consumer a Fock build puts after the integrals) served among ERI and
+    derivative requests: J and K match the oracle's full-tensor J and K, and
+    the same request alone and in the mix gives the same matrices (up to
+    the atomic sums' order).  Screening at 1e-10 drops quartets the oracle
+    keeps, so the bar against it is 1e-9 absolute."""
+    from test_gpu_jk import _density
+
+    def mk():
+        shells_a = workloads.config2(natom=2, seed=21)
+        shells_b = workloads.config2(natom=3, seed=22)
+        rs = [serving.Request(0, 2, 21, "jk", 0.0, shells=shells_a),
+              serving.Request(1, 12, 23, "eri", 0.0),
+              serving.Request(2, 3, 22, "jk", 0.0, shells=shells_b),
+              serving.Request(3, 10, 24, "deriv", 0.0)]
+        for r in rs[::2]:
+            nbf = sum(simint.ncart(s[0]) for s in r.shells)
+            r.density = _density(nbf, r.seed)
+        return rs
+
+    alone = {}
+    for r in mk()[::2]:
+        srv = serving.Server(hbm_budget=8 << 30, slots=1, tol=1e-10, chunk_doubles=1 << 20)
+        srv.serve([r])
+        assert r.error is None, r.error
+        alone[r.rid] = (r.J, r.K)
+    mix = mk()
+    for i, r in enumerate(mix):
+        r.arrival = 0.02 * i
+    srv = serving.Server(hbm_budget=8 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 20)
+    srv.serve(mix)
+    for r in mix:
+        assert r.error is None, r.error
+    for r in mix[::2]:
+        sa = normalized_arrays(r.shells)
+        g, _ = oracle.eri_tensor(sa)
+        Jr, Kr = oracle.jk(g, r.density)
+        ej, ek = np.abs(r.J - Jr).max(), np.abs(r.K - Kr).max()
+        print("request %d: |dJ| %.2e |dK| %.2e (max |J| %.2e, |K| %.2e)" % (r.rid, ej, ek, np.abs(Jr).max(),
+                                                                            np.abs(Kr).max()))
+        assert ej <= 1e-9 and ek <= 1e-9
+        J0, K0 = alone[r.rid]
+        assert np.abs(r.J - J0).max() <= 1e-12 * np.abs(J0).max()
+        assert np.abs(r.K - K0).max() <= 1e-12 * np.abs(K0).max()
