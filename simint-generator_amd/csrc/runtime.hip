@@ -38,10 +38,19 @@ inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 // the class and each explicit centre slot j (the three centres other than
 // the skipped one, in A..D order) and direction x, the offsets of the
 // (a+1) and (a-1) values in their class blocks and the factor a_x
-struct D1Entry {
-    int ip, im, ex, pad;
-};
+// One entry of a derivative assembly table, packed into 32 bits (the
+// assembly reads nine per Cartesian): bits 0-14 the (a+1) index, 15-29 the
+// (a-1) index, 30-31 the exponent the (a-1) term is weighted by.  Indices
+// stay below 2^15: the largest (a+1) block, (gf|ff), has 15000 values.
+typedef unsigned int D1Entry;
+__host__ __device__ inline D1Entry d1_pack(int ip, int im, int ex)
+{
+    return (unsigned)ip | ((unsigned)im << 15) | ((unsigned)ex << 30);
+}
 constexpr int D1_N = SIMINT_OSTEI_DERIV1_MAXAM + 1;
+static_assert((D1_N + 1) * (D1_N + 2) / 2 * (D1_N * (D1_N + 1) / 2) * (D1_N * (D1_N + 1) / 2) *
+                      (D1_N * (D1_N + 1) / 2) < (1 << 15),
+              "derivative assembly indices must fit the 15-bit fields of D1Entry");
 // the (a+1) classes of a derivative reach one AM above the derivative maximum
 constexpr int kTermMaxAM = SIMINT_OSTEI_DERIV1_MAXAM + 1;
 
@@ -125,16 +134,17 @@ void device_init()
                                 ep[x] += 1;
                                 dp[cen] = ncart_h(l + 1);
                                 cp[cen] = cidx(ep[0], ep[1], ep[2]);
-                                D1Entry t{((cp[0] * dp[1] + cp[1]) * dp[2] + cp[2]) * dp[3] + cp[3], 0, e[x], 0};
+                                const int ip = ((cp[0] * dp[1] + cp[1]) * dp[2] + cp[2]) * dp[3] + cp[3];
+                                int im = 0;
                                 if (e[x] > 0) {
                                     int dm[4] = {n[0], n[1], n[2], n[3]}, cm[4] = {comp[0], comp[1], comp[2], comp[3]};
                                     int em[3] = {e[0], e[1], e[2]};
                                     em[x] -= 1;
                                     dm[cen] = ncart_h(l - 1);
                                     cm[cen] = cidx(em[0], em[1], em[2]);
-                                    t.im = ((cm[0] * dm[1] + cm[1]) * dm[2] + cm[2]) * dm[3] + cm[3];
+                                    im = ((cm[0] * dm[1] + cm[1]) * dm[2] + cm[2]) * dm[3] + cm[3];
                                 }
-                                all.push_back(t);
+                                all.push_back(d1_pack(ip, im, e[x]));
                             }
                         }
                     }
@@ -478,9 +488,20 @@ __global__ void __launch_bounds__(256) deriv1_assemble(const Deriv1Args d)
     const long long gid = gid0 + threadIdx.x;
     const bool live = gid < total;
     double v12[12];
+    // (quartet, Cartesian) of this thread: a 32-bit division where it fits
+    long long q = 0;
+    int k = 0;
     if (live) {
-        const long long q = gid / d.nk;
-        const int k = (int)(gid - q * d.nk);
+        if (total < (1LL << 31)) {
+            const unsigned qq = (unsigned)gid / (unsigned)d.nk;
+            q = qq;
+            k = (int)((unsigned)gid - qq * (unsigned)d.nk);
+        } else {
+            q = gid / d.nk;
+            k = (int)(gid - q * d.nk);
+        }
+    }
+    if (live) {
         const D1Entry *__restrict__ t = d.tab + (long long)k * 9;
         double s3[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -490,8 +511,9 @@ __global__ void __launch_bounds__(256) deriv1_assemble(const Deriv1Args d)
 #pragma unroll
             for (int x = 0; x < 3; x++) {
                 const D1Entry e = t[j * 3 + x];
-                double v = P[e.ip];
-                if (e.ex > 0) v -= (double)e.ex * M[e.im];
+                const int ex = (int)(e >> 30);
+                double v = P[e & 0x7fffu];
+                if (ex > 0) v -= (double)ex * M[(e >> 15) & 0x7fffu];
                 v12[j * 3 + x] = v;
                 s3[x] -= v;
             }
@@ -501,8 +523,6 @@ __global__ void __launch_bounds__(256) deriv1_assemble(const Deriv1Args d)
     }
     if (d.qout) {
         if (!live) return;
-        const long long q = gid / d.nk;
-        const int k = (int)(gid - q * d.nk);
         double *__restrict__ o = d.out + d.qout[q] + (long long)k * 12;
 #pragma unroll
         for (int j = 0; j < 3; j++)
