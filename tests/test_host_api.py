@@ -121,3 +121,27 @@ def test_worksize_table():
     assert L.simint_eri_worksize(0, 1) == 2 * 81 + 81 + 149
     assert L.simint_eri_worksize(0, 2) == 2 * 961 + 4332 + 2405
     assert L.simint_eri_worksize(1, 1) > L.simint_eri_worksize(0, 1)
+
+
+def test_device_entry_points_reject_bad_arguments_without_a_device():
+    """The device-resident entry points check their arguments before they
+    touch the device: NULL pair sets, a missing density or function table,
+    an out-of-range bra range or precision are SIMINT_ERROR_ARGUMENT, here
+    as on a GPU box; with valid-looking arguments and no GPU the answer is
+    SIMINT_ERROR_NO_DEVICE, never a CPU result."""
+    L = simint.lib()
+    jk = simint.JK(4, 0, None, None, None, None, None)
+    one = (ctypes.c_int * 1)(1)
+    off = (ctypes.c_longlong * 1)(0)
+    assert L.simint_hip_compute_jk_range(None, None, 0, 1, None, one, off, 0.0, 0, None,
+                                         ctypes.byref(jk)) == simint.ERROR_ARGUMENT
+    assert L.simint_hip_compute_jk_range(None, None, 0, 1, None, one, off, 0.0, 0, None, None) \
+        == simint.ERROR_ARGUMENT
+    assert L.simint_hip_jk_symmetrize(0, None, None) == simint.ERROR_ARGUMENT
+    assert L.simint_hip_compute_eri_range(None, None, 0, 1, None, one, off, 0.0, 0, None) == simint.ERROR_ARGUMENT
+    assert L.simint_hip_pair_bounds(None, None) == simint.ERROR_ARGUMENT
+    prev = L.simint_hip_set_jk_rows(0)
+    assert L.simint_hip_set_jk_rows(prev) == 0
+    import torch
+    if not torch.cuda.is_available():
+        assert L.simint_hip_jk_symmetrize(4, None, None) == simint.ERROR_NO_DEVICE
