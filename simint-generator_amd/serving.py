@@ -226,6 +226,9 @@ class Server:
 
     def _setup(self, r):
         import torch
+        if r.kind not in ("eri", "deriv", "jk"):
+            r.error = "rejected: unknown request kind %r" % (r.kind,)
+            return
         shells = self._shells(r)
         bad = validate(shells)
         if bad is not None:

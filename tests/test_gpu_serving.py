@@ -84,8 +84,9 @@ def test_admission_respects_budget(gpu):
 
 
 def test_malformed_and_timed_out_requests_leave_others_intact(gpu):
-    """A NaN exponent, an f shell and a zero time limit end those requests
-    with an error; the requests beside them give the same result as alone."""
+    """A NaN exponent, an f shell, a zero time limit, an unknown kind and a
+    density of the wrong shape end those requests with an error; the
+    requests beside them give the same result as alone."""
     good = [serving.Request(0, 8, 51, "eri", 0.0), serving.Request(1, 6, 52, "deriv", 0.0)]
     alone = {}
     for r in good:
@@ -97,15 +98,20 @@ def test_malformed_and_timed_out_requests_leave_others_intact(gpu):
     am, xyz, al, co = nan_shells[3]
     nan_shells[3] = (am, xyz, [float("nan")] + list(al[1:]), co)
     f_shells = workloads.config2(natom=2, seed=54) + [(3, (0.0, 0.0, 0.0), [1.0], [1.0])]
+    bad_density = np.ones((3, 3))
     mix = [serving.Request(0, 8, 51, "eri", 0.0), serving.Request(1, 6, 52, "deriv", 0.01),
            serving.Request(2, 2, 0, "eri", 0.0, shells=nan_shells),
            serving.Request(3, 2, 0, "eri", 0.02, shells=f_shells),
-           serving.Request(4, 10, 55, "eri", 0.0, timeout=0.0)]
+           serving.Request(4, 10, 55, "eri", 0.0, timeout=0.0),
+           serving.Request(5, 2, 56, "hessian", 0.0),
+           serving.Request(6, 2, 57, "jk", 0.0, density=bad_density)]
     srv = serving.Server(hbm_budget=8 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 22)
     srv.serve(mix)
     assert mix[2].error.startswith("rejected") and "exponent" in mix[2].error
     assert mix[3].error.startswith("rejected") and "angular momentum" in mix[3].error
     assert mix[4].error.startswith("timeout")
+    assert mix[5].error.startswith("rejected") and "kind" in mix[5].error
+    assert mix[6].error.startswith("rejected") and "density" in mix[6].error
     for r in mix[:2]:
         assert r.error is None, r.error
         assert (r.digest, r.quartets) == alone[r.rid], r.rid
