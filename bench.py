@@ -250,7 +250,7 @@ def mixed_system_leg(args):
     import quartets
     import simint
     import workloads
-    shells = workloads.config2(natom=args.natom, seed=0)
+    shells = workloads.grid_molecule(natom=args.natom, seed=0)
     t0 = time.perf_counter()
     ss = quartets.ScreenedSet(shells, tol=1e-12)
     plan64 = ss.launches
@@ -408,7 +408,7 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
     import workloads
     info = json.load(open(os.path.join(PKG, "csrc", "gen", "classes.json")))
     t0 = time.perf_counter()
-    shells, q = workloads.config3(nquartet=nquartet, seed=0)
+    shells, q = workloads.all_class_stream(nquartet=nquartet, seed=0)
     basis = simint.Basis(shells)
     b = batch.QuartetBatch(basis, q)
     setup = time.perf_counter() - t0
@@ -474,7 +474,7 @@ def deriv_leg(natom=64, tol=1e-12):
     import quartets
     import simint
     import workloads
-    shells = workloads.config2(natom=natom, seed=0)
+    shells = workloads.grid_molecule(natom=natom, seed=0)
     ss = quartets.ScreenedSet(shells, tol=tol, out_doubles=1 << 27, deriv=1)
     work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
     out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
@@ -508,7 +508,7 @@ def fp64_64atom_leg():
     import quartets
     import simint
     import workloads
-    ss = quartets.ScreenedSet(workloads.config2(natom=64, seed=0), tol=1e-12)
+    ss = quartets.ScreenedSet(workloads.grid_molecule(natom=64, seed=0), tol=1e-12)
     out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
     ss.run_launches(out.data_ptr(), shares(ss.launches, 4)[0])
     simint.synchronize()

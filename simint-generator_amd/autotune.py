@@ -37,7 +37,7 @@ def main():
     import simint
     import workloads
     L = simint.lib()
-    ss = quartets.ScreenedSet(workloads.config2(natom=a.natom, seed=0), tol=1e-12)
+    ss = quartets.ScreenedSet(workloads.grid_molecule(natom=a.natom, seed=0), tol=1e-12)
     out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
     stream = torch.cuda.ExternalStream(L.simint_hip_stream())
     by = {}
@@ -87,7 +87,7 @@ def tune_lists(a):
     import simint
     import workloads
     L = simint.lib()
-    shells, q = workloads.config3(nquartet=a.nquartet, seed=1)
+    shells, q = workloads.all_class_stream(nquartet=a.nquartet, seed=1)
     basis = simint.Basis(shells)
     b = batch.QuartetBatch(basis, q)
     out = torch.empty(b.out_doubles, dtype=torch.float64, device="cuda")

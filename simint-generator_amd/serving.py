@@ -168,7 +168,7 @@ class Server:
         return time.perf_counter() - self.t0
 
     def _shells(self, r):
-        return r.shells if r.shells is not None else workloads.config2(natom=r.natom, seed=r.seed)
+        return r.shells if r.shells is not None else workloads.grid_molecule(natom=r.natom, seed=r.seed)
 
     @staticmethod
     def _nshell(r):

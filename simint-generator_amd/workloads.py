@@ -1,19 +1,19 @@
-"""Synthetic shell sets for the benchmark configurations.
+"""Synthetic shell sets of the benchmark and test workloads.
 
 Random numbers come from SplitMix64 (Steele, Lea, Flood 2014) so that the
 same stream can be reproduced by any host code: u = (next() >> 11) * 2^-53.
 Log-uniform means exp(U(ln lo, ln hi)).  Draw order per shell: centre
 x, y, z (or per atom), then the exponents, then the coefficients.
 
-  config1(seed=0): 40 p shells of contraction depth 3, exponents log-uniform
+  pshell_quartets(seed=0): 40 p shells of contraction depth 3, exponents log-uniform
       in [0.1, 20], coefficients uniform in [0.1, 1], centres uniform in a
       10-bohr cube; 1000 quartets = 1000 draws of four shell indices.
-  config2(natom=64, seed=0): atoms on a jittered cubic grid (spacing 3 bohr,
+  grid_molecule(natom=64, seed=0): atoms on a jittered cubic grid (spacing 3 bohr,
       jitter U[-0.3, 0.3] per coordinate), each with s(6), s(3), p(3) core
       shells (exponents log-uniform in [0.1, 5000]) and s(1), p(1), d(1)
       valence shells (log-uniform in [0.05, 2]); coefficients U[0.1, 1].
       natom=64 is a 4x4x4 grid, natom=256 an 8x8x4 grid.
-  config3(nquartet=10**6, seed=0): a pool of 4096 shells, 1024 of each AM
+  all_class_stream(nquartet=10**6, seed=0): a pool of 4096 shells, 1024 of each AM
       0..3 (in that order); per shell the contraction depth U{1..6}, the
       centre uniform in a 30-bohr cube, exponents log-uniform in
       [0.05, 1e4], coefficients U[0.1, 1]; then nquartet quartets of four
@@ -61,7 +61,7 @@ class SplitMix64:
         return np.minimum((u * n).astype(np.int64), n - 1)
 
 
-def config1(seed=0, nshell=40, nquartet=1000):
+def pshell_quartets(seed=0, nshell=40, nquartet=1000):
     r = SplitMix64(seed)
     shells = []
     for _ in range(nshell):
@@ -88,8 +88,8 @@ def grid_dims(natom):
     return (n, n, n)
 
 
-def config2(natom=64, seed=0, spacing=3.0, jitter=0.3):
-    """Shells of the config-2 recipe, returned as a list ordered atom by
+def grid_molecule(natom=64, seed=0, spacing=3.0, jitter=0.3):
+    """Shells of the grid-molecule recipe, returned as a list ordered atom by
     atom: (am, xyz, alphas, coefs)."""
     r = SplitMix64(seed)
     nx, ny, nz = grid_dims(natom)
@@ -115,7 +115,7 @@ def config2(natom=64, seed=0, spacing=3.0, jitter=0.3):
     return shells
 
 
-def config3_pool(seed=0, per_am=1024, maxam=3, cube=30.0):
+def mixed_pool(seed=0, per_am=1024, maxam=3, cube=30.0):
     r = SplitMix64(seed)
     shells = []
     for am in range(maxam + 1):
@@ -128,9 +128,9 @@ def config3_pool(seed=0, per_am=1024, maxam=3, cube=30.0):
     return shells, r
 
 
-def config3(nquartet=10 ** 6, seed=0):
+def all_class_stream(nquartet=10 ** 6, seed=0):
     """(shells, quartets[nquartet, 4]) of the mixed-class stream."""
-    shells, r = config3_pool(seed)
+    shells, r = mixed_pool(seed)
     q = r.indices(4 * nquartet, len(shells)).reshape(nquartet, 4)
     return shells, q
 

@@ -8,7 +8,7 @@ max|gpu - oracle| <= max(1e-14 max|oracle|, 4 max|oracle - exact|): the GPU
 may differ from the reference algorithm by no more than that algorithm's own
 FP64 rounding error.  A flat 1e-14 normwise bar per quartet is not attainable
 in FP64: the oracle itself is up to 4.6e-13 (normwise) away from the exact
-algebra on config 1, from cancellation in quartets whose integrals are tiny
+algebra on the 1000 (pp|pp) quartets, from cancellation in quartets whose integrals are tiny
 (absolute error <= 2e-16); the normwise statistics are printed.
 """
 import itertools
@@ -62,20 +62,20 @@ def _check(sa, basis, A, B, C, D, screen=0, tol=0.0, exact=False, deriv=0):
     return st
 
 
-def test_config1_pppp_single_quartets(gpu):
-    """1000 (pp|pp) quartets, one simint_compute_eri call each."""
-    shells, quartets = workloads.config1()
+def test_pppp_1000_single_quartets(gpu):
+    """1000 (pp|pp) quartets of 40 p shells, one simint_compute_eri call each."""
+    shells, quartets = workloads.pshell_quartets()
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     worst = 0.0
     for (a, b, c, d) in quartets:
         st = _check(sa, basis, [a], [b], [c], [d], exact=True)
         worst = max(worst, st["max_normwise"])
-    print("config1 worst normwise rel vs oracle", worst)
+    print("pshell_quartets worst normwise rel vs oracle", worst)
 
 
-def test_config1_pppp_batched(gpu):
-    shells, _ = workloads.config1()
+def test_pppp_1000_batched(gpu):
+    shells, _ = workloads.pshell_quartets()
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     st = _check(sa, basis, list(range(0, 6)), list(range(0, 6)), list(range(10, 25)), list(range(20, 40)))
@@ -177,7 +177,7 @@ def test_deriv1_all_classes(gpu, mol):
 def test_deriv1_tight_exponents(gpu):
     """The 64-atom recipe's shells (core exponents up to 5000) on two atoms,
     all classes up to d: the ill-conditioned derivatives."""
-    shells = workloads.config2(natom=2, seed=5)
+    shells = workloads.grid_molecule(natom=2, seed=5)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}
@@ -192,7 +192,7 @@ def test_deriv1_screened_and_limits(gpu, screen):
     """Primitive screening on the derivative path, the same-shell pair
     weighting (alpha2 = beta2 = a_i + a_j on a triangle pair), and a class
     above the derivative maximum rejected with SIMINT_ERROR_UNSUPPORTED."""
-    shells = workloads.config2(natom=2, seed=5)
+    shells = workloads.grid_molecule(natom=2, seed=5)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}
@@ -240,7 +240,7 @@ def test_lane_and_wave_kernels_agree(gpu):
     """Every kernel family (lane-per-quartet in registers, wave-per-quartet
     in LDS, the split lane-per-quartet kernels) against the oracle for every
     class up to d that has it."""
-    shells = workloads.config2(natom=2)
+    shells = workloads.grid_molecule(natom=2)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}
@@ -265,7 +265,7 @@ def test_primitive_screening(gpu, screen, tol):
     """Primitive screening (OSTEI_Writer.cpp:504-587, one primitive quartet at
     a time) matches the oracle's restatement, including the primitive sort the
     screened pairs get (shell.c:579-581)."""
-    shells = workloads.config2(natom=2, seed=3)
+    shells = workloads.grid_molecule(natom=2, seed=3)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}
@@ -276,7 +276,7 @@ def test_primitive_screening(gpu, screen, tol):
 
 
 def test_empty_and_clipped(gpu):
-    shells, _ = workloads.config1()
+    shells, _ = workloads.pshell_quartets()
     basis = simint.Basis(shells)
     P = simint.Pairs(basis, [0, 1], [2, 3])
     Q = simint.Pairs(basis, [4], [5])
@@ -314,7 +314,7 @@ def test_device_prefix_and_diag(gpu):
     """simint_hip_compute_eri_prefix / _diag: device-resident pairs, per-bra
     ket prefixes, integrals left in HBM."""
     import torch
-    shells = workloads.config2(natom=3, seed=1)
+    shells = workloads.grid_molecule(natom=3, seed=1)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}
@@ -360,7 +360,7 @@ def test_split_kernels_at_scale(gpu):
     system through each family, equal to the oracle on a sample and to each
     other everywhere."""
     import torch
-    shells = workloads.config2(natom=16, seed=0)
+    shells = workloads.grid_molecule(natom=16, seed=0)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     L = simint.lib()

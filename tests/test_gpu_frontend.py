@@ -30,7 +30,7 @@ def test_device_pair_fill_matches_oracle(gpu, screen):
     exp()/pow() -- the prefactor and the screening value -- which may differ
     in the last two bits; the descending sort by screening value gives the
     same primitive order."""
-    shells = workloads.config2(natom=1, seed=5) + [(3, (0.4, -0.3, 0.2), [3.0, 0.6, 0.15], [0.3, 0.6, 0.4])]
+    shells = workloads.grid_molecule(natom=1, seed=5) + [(3, (0.4, -0.3, 0.2), [3.0, 0.6, 0.15], [0.3, 0.6, 0.4])]
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     ds = simint.DeviceShells(basis)
@@ -68,7 +68,7 @@ def test_device_front_end_screened_set(gpu):
     bounds, sort, counts) holds the same quartets as the host-side set up,
     and its integrals match the oracle on a sample."""
     import torch
-    shells = workloads.config2(natom=12, seed=7)
+    shells = workloads.grid_molecule(natom=12, seed=7)
     host = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24)
     dev = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24, front_end="device")
     print("set-up s: host %.3f, device %.3f" % (host.setup_s, dev.setup_s))

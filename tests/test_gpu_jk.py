@@ -103,7 +103,7 @@ def test_jk_f_shells_small_chunks(gpu):
 def test_jk_tight_exponents(gpu):
     """Two atoms of the 64-atom recipe (core exponents up to 5000, depth-6
     contractions)."""
-    assert _compare(workloads.config2(natom=2, seed=7), 3)
+    assert _compare(workloads.grid_molecule(natom=2, seed=7), 3)
 
 
 def test_jk_only_one_of_them(gpu):
@@ -126,7 +126,7 @@ def test_jk_exchange_paths(gpu, rows):
     L = simint.lib()
     prev = L.simint_hip_set_jk_rows(rows)
     try:
-        assert _compare(_f_set(seed=9) + workloads.config2(natom=1, seed=9), 5)
+        assert _compare(_f_set(seed=9) + workloads.grid_molecule(natom=1, seed=9), 5)
     finally:
         L.simint_hip_set_jk_rows(prev)
 
@@ -137,7 +137,7 @@ def test_jk_mixed_plan(gpu):
     and K within 1e-9 absolute of the oracle's -- each FP32 integral is
     within the mixed path's 1e-10, and a J or K element sums such errors
     weighted by density elements of at most ~0.1."""
-    shells = workloads.config2(natom=2, seed=11)
+    shells = workloads.grid_molecule(natom=2, seed=11)
     sa = normalized_arrays(shells)
     g, _ = oracle.eri_tensor(sa)
     D = _density(g.shape[0], 6)

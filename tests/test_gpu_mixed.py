@@ -32,7 +32,7 @@ def _chunk_outputs(ss, chunk, out):
 
 def test_mixed_vs_fp64_and_oracle(gpu):
     import torch
-    shells = workloads.config2(natom=8, seed=2)
+    shells = workloads.grid_molecule(natom=8, seed=2)
     ss64 = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24)
     ssm = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24, mixed=MIXED_BOUND)
     assert ss64.nchunk == ssm.nchunk and ss64.total_quartets == ssm.total_quartets
@@ -93,7 +93,7 @@ def test_range_matches_prefix(gpu):
     """Ket ranges [lo, hi) per bra in FP64: the same values at the same
     places as the prefix call over [0, hi)."""
     import torch
-    shells = workloads.config2(natom=3, seed=4)
+    shells = workloads.grid_molecule(natom=3, seed=4)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}

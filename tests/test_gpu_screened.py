@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 def test_screened_pass_fp64_vs_oracle(gpu):
     import torch
-    shells = workloads.config2(natom=12, seed=5)
+    shells = workloads.grid_molecule(natom=12, seed=5)
     ss = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24)
     sa = normalized_arrays(shells)
     sample = ss.located_sample(600, seed=1)
@@ -55,7 +55,7 @@ def test_screened_pass_fp64_vs_oracle(gpu):
 def test_dropin_concurrent_host_threads(gpu):
     """Four threads, each computing (P_t | Q) for its own bra pairs P_t and
     one shared ket set Q, eight times in a row, all at once."""
-    shells = workloads.config2(natom=4, seed=9)
+    shells = workloads.grid_molecule(natom=4, seed=9)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}

@@ -54,7 +54,7 @@ def test_served_results_vs_oracle(gpu):
     for r in reqs:
         assert r.error is None, r.error
         assert len(r.samples) == 48
-        sa = normalized_arrays(workloads.config2(natom=r.natom, seed=r.seed))
+        sa = normalized_arrays(workloads.grid_molecule(natom=r.natom, seed=r.seed))
         qs = [q for q, _ in r.samples]
         got = np.concatenate([v for _, v in r.samples])
         ref, offs, sizes = oracle.eri(sa, qs, deriv=1 if r.kind == "deriv" else 0)
@@ -73,7 +73,7 @@ def test_admission_respects_budget(gpu):
     is admitted on its estimate and then charged what its set-up allocated."""
     reqs = [serving.Request(i, 6, 30 + i, "eri", 0.0) for i in range(3)]
     est = serving.footprint(36, "eri", 1 << 22)
-    ss = quartets.ScreenedSet(workloads.config2(natom=6, seed=30), tol=1e-10, out_doubles=1 << 22)
+    ss = quartets.ScreenedSet(workloads.grid_molecule(natom=6, seed=30), tol=1e-10, out_doubles=1 << 22)
     act = ss.device_bytes()
     ss.free()
     assert act > 0
@@ -94,10 +94,10 @@ def test_malformed_and_timed_out_requests_leave_others_intact(gpu):
         srv.serve([r])
         assert r.error is None, r.error
         alone[r.rid] = (r.digest, r.quartets)
-    nan_shells = workloads.config2(natom=2, seed=53)
+    nan_shells = workloads.grid_molecule(natom=2, seed=53)
     am, xyz, al, co = nan_shells[3]
     nan_shells[3] = (am, xyz, [float("nan")] + list(al[1:]), co)
-    f_shells = workloads.config2(natom=2, seed=54) + [(3, (0.0, 0.0, 0.0), [1.0], [1.0])]
+    f_shells = workloads.grid_molecule(natom=2, seed=54) + [(3, (0.0, 0.0, 0.0), [1.0], [1.0])]
     bad_density = np.ones((3, 3))
     mix = [serving.Request(0, 8, 51, "eri", 0.0), serving.Request(1, 6, 52, "deriv", 0.01),
            serving.Request(2, 2, 0, "eri", 0.0, shells=nan_shells),
@@ -137,7 +137,7 @@ def test_screened_deriv_pass_vs_oracle(gpu):
     """A first-derivative pass of the screened set (the serving deriv path)
     against the oracle on a sample of its quartets."""
     import torch
-    shells = workloads.config2(natom=3, seed=21)
+    shells = workloads.grid_molecule(natom=3, seed=21)
     ss = quartets.ScreenedSet(shells, tol=1e-10, out_doubles=1 << 22, deriv=1)
     work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
     out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
@@ -180,8 +180,8 @@ def test_jk_requests_vs_oracle_alone_and_in_mix(gpu):
     from test_gpu_jk import _density
 
     def mk():
-        shells_a = workloads.config2(natom=2, seed=21)
-        shells_b = workloads.config2(natom=3, seed=22)
+        shells_a = workloads.grid_molecule(natom=2, seed=21)
+        shells_b = workloads.grid_molecule(natom=3, seed=22)
         rs = [serving.Request(0, 2, 21, "jk", 0.0, shells=shells_a),
               serving.Request(1, 12, 23, "eri", 0.0),
               serving.Request(2, 3, 22, "jk", 0.0, shells=shells_b),

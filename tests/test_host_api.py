@@ -53,7 +53,7 @@ def test_struct_layout():
 
 
 def test_normalization_matches_oracle():
-    shells = mol_shells("water.6-31gss") + workloads.config2(natom=1)
+    shells = mol_shells("water.6-31gss") + workloads.grid_molecule(natom=1)
     basis = simint.Basis(shells)
     sa = oracle.ShellArrays(shells)
     oracle.normalize(sa)
@@ -63,7 +63,7 @@ def test_normalization_matches_oracle():
 
 @pytest.mark.parametrize("screen", [simint.SCREEN_NONE, simint.SCREEN_FASTSCHWARZ])
 def test_pair_fill_matches_oracle(screen):
-    shells = workloads.config2(natom=1, seed=5)
+    shells = workloads.grid_molecule(natom=1, seed=5)
     basis = simint.Basis(shells)
     sa = oracle.ShellArrays(shells)
     oracle.normalize(sa)
@@ -91,7 +91,7 @@ def test_pair_fill_matches_oracle(screen):
 
 
 def test_cat_shellpairs():
-    shells = workloads.config2(natom=1, seed=2)
+    shells = workloads.grid_molecule(natom=1, seed=2)
     basis = simint.Basis(shells)
     singles = [simint.Pairs(basis, [i], [j], simint.SCREEN_FASTSCHWARZ) for i, j in [(0, 1), (3, 3), (2, 0)]]
     arr = (ctypes.POINTER(simint.MultiShellpair) * 3)(*[ctypes.pointer(p.mp) for p in singles])

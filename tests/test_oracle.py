@@ -152,11 +152,11 @@ def _cases():
             continue
         qs.append(tuple(int(rng.choice(byam[a])) for a in am))
     out.append(("water.aug-cc-pvtz (to f)", shells, qs))
-    shells = workloads.config2(natom=2, seed=7)
+    shells = workloads.grid_molecule(natom=2, seed=7)
     qs = [tuple(int(x) for x in rng.integers(0, len(shells), 4)) for _ in range(300)]
-    out.append(("config2 2 atoms", shells, qs))
-    shells, qs1 = workloads.config1()
-    out.append(("config1", shells, qs1[:300]))
+    out.append(("grid_molecule 2 atoms", shells, qs))
+    shells, qs1 = workloads.pshell_quartets()
+    out.append(("pshell_quartets", shells, qs1[:300]))
     return out
 
 
@@ -181,7 +181,7 @@ def _block(out, offs, sizes, k, shape):
 
 
 def test_invariants():
-    shells = workloads.config2(natom=2, seed=3)
+    shells = workloads.grid_molecule(natom=2, seed=3)
     sa = normalized_arrays(shells)
     rng = np.random.default_rng(2)
     nc = [(s[0] + 1) * (s[0] + 2) // 2 for s in shells]
@@ -213,7 +213,7 @@ def test_invariants():
 
 
 def test_deriv1_against_finite_differences():
-    shells = workloads.config2(natom=2, seed=11)
+    shells = workloads.grid_molecule(natom=2, seed=11)
     sa = normalized_arrays(shells)
     rng = np.random.default_rng(5)
     qs = [tuple(int(x) for x in rng.integers(0, len(shells), 4)) for _ in range(25)]

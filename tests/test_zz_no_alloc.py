@@ -29,7 +29,7 @@ def _noalloc(P, Q, n, qb, qk, qo, out):
 
 def test_noalloc_matches_work_buffer_entry(gpu):
     import torch
-    shells = workloads.config2(natom=2, seed=17)
+    shells = workloads.grid_molecule(natom=2, seed=17)
     basis = simint.Basis(shells)
     sa = normalized_arrays(shells)
     byam = {}
