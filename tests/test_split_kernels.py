@@ -1,0 +1,59 @@
+"""Generated lane-per-quartet and split kernels, checked on the CPU from
+their source text: every contracted input is accumulated exactly once per
+primitive quartet, and the ket-top split kernels' local-data-share use (the
+per-primitive-quartet setup cache, the output rows and their offsets) stays
+within one wavefront's share of a CU, with every cache access inside the
+declared array and guarded by the same slot count on both sides."""
+import re
+
+import pytest
+
+from emitter import generate, hip_lpq, hip_lpqs
+from emitter.rrgraph import graph
+
+LPQ_CLASSES = [(1, 0, 0, 0), (1, 1, 1, 1), (2, 1, 1, 0), (2, 2, 1, 0), (2, 0, 2, 0)]
+
+
+@pytest.mark.parametrize("am", LPQ_CLASSES)
+def test_lpq_accumulates_every_top_once(am):
+    src = hip_lpq.emit_lpq(am)
+    body = src[src.index("for (int j = 0; j < nq; ++j)"):src.index("if (lane < wi.lane_lo")]
+    ntop = len(graph(*am).top_keys)
+    for t in range(ntop):
+        assert len(re.findall(r"\bacc%d \+= " % t, body)) == 1, (am, t)
+    assert not re.search(r"\bacc%d \+= " % ntop, body)
+
+
+def _split_classes():
+    out = []
+    for am in generate.classes(2):
+        for G in generate.lpqsf_variants(am):
+            out.append((am, G))
+    return out
+
+
+@pytest.mark.parametrize("am,G", _split_classes())
+def test_ket_top_split_kernel_lds_and_cache(am, G):
+    src = hip_lpqs.emit_lpqs(am, G, "f")
+    g = graph(*am)
+    # declared LDS arrays and their sizes (doubles / long longs, 8 bytes each)
+    sizes = {}
+    for m in re.finditer(r"__shared__ (double|long long) (\w+)\[([^\]]+)\];", src):
+        sizes[m.group(2)] = eval(m.group(3))
+    total = 8 * sum(sizes.values())
+    assert total <= hip_lpqs.LDS_PER_WAVE, (am, G, sizes)
+    KC = hip_lpqs.setup_cache_slots(am) if G > 1 else 0
+    NV = g.Ltot + 10
+    if KC == 0:
+        assert "pc[" not in src
+        return
+    assert sizes["pc"] == KC * NV * 64
+    # every access is pc[(t * NV + v) * 64 + lane] with 0 <= v < NV, inside
+    # a branch on t < KC
+    acc = re.findall(r"pc\[\(t \* (\d+) \+ (\d+)\) \* 64 \+ lane\]", src)
+    assert acc and all(int(nv) == NV and 0 <= int(v) < NV for nv, v in acc)
+    assert len(re.findall(r"pc\[", src)) == len(acc) + 1   # and the declaration
+    writes = re.findall(r"pc\[\(t \* \d+ \+ (\d+)\) \* 64 \+ lane\] = ", src)
+    assert sorted(int(v) for v in writes) == list(range(NV))          # group 0 writes every value once
+    guards = re.findall(r"if \(t < (\d+)\)", src)
+    assert guards and all(int(k) == KC for k in guards) and len(guards) == G   # group 0 + G-1 readers
