@@ -18,7 +18,10 @@
 // bra-side J sums are reduced across the wave before one atomic add per
 // element; the ket-side J and the four exchange blocks are added per lane.
 // The exchange blocks whose row is a bra function (K_ac, K_ad) are summed
-// over b (and K_bc, K_bd over a) in registers before their atomic adds.
+// over b (and K_bc, K_bd over a) in registers before their atomic adds, and
+// where the bra's rows fit the local data share one block takes all of a
+// bra's items and sums those rows over all its kets there first
+// (jk_digest_rows).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
