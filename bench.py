@@ -38,6 +38,8 @@ Besides the metric the line carries:
   fp64_64atom       the 64-atom system in FP64 (the all-classes-to-d config),
                     and the same pass with the Coulomb/exchange digestion
                     after every launch (fp64_64atom.jk);
+  dropin_pcie       the host-buffer drop-in call on one large batch (its
+                    rate includes the PCIe copies, which `value` does not);
   cpu_baseline      the CPU oracle (a C restatement of the reference
                     algorithm) on every host core the box gives this job,
                     on a sample of the same quartets.
@@ -502,6 +504,31 @@ def deriv_leg(natom=64, tol=1e-12):
     return res
 
 
+def dropin_leg():
+    """The host-buffer drop-in call (simint_compute_eri: pairs packed and
+    copied to the device, integrals copied back over PCIe) on one large
+    batch: P = Q = every (p p) shell pair of 16 atoms of the 64-atom recipe.
+    Its rate includes both host<->device copies; `value` does not."""
+    import simint
+    import workloads
+    shells = workloads.grid_molecule(natom=16, seed=0)
+    basis = simint.Basis(shells)
+    ps = [i for i, s in enumerate(shells) if s[0] == 1]
+    pairs = [(a, b) for ia, a in enumerate(ps) for b in ps[:ia + 1]]
+    P = simint.Pairs.from_pairs(basis, pairs)
+    simint.compute_eri(P, P)          # warm-up: device set-up and staging buffers
+    t = time.perf_counter()
+    rc, out = simint.compute_eri(P, P)
+    wall = time.perf_counter() - t
+    nq = len(pairs) ** 2
+    res = {"what": "simint_compute_eri, host buffers, P = Q = %d (pp) pairs of 16 atoms of the 64-atom recipe"
+                   % len(pairs), "quartets": nq, "returned": int(rc), "s": wall, "value": nq / wall,
+           "unit": "shell quartets/s", "GB_to_host": 8.0 * out.size / 1e9}
+    P.free()
+    basis.free()
+    return res
+
+
 def fp64_64atom_leg():
     """The 64-atom all-classes-to-d system in plain FP64, one full pass."""
     import torch
@@ -675,6 +702,8 @@ def main():
     if not args.no_fp64_64 and not args.only_class:
         line["fp64_64atom"] = fp64_64atom_leg()
         progress("64-atom FP64 leg done")
+        line["dropin_pcie"] = dropin_leg()
+        progress("drop-in leg done")
     print(json.dumps(line))
 
 
