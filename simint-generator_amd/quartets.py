@@ -17,6 +17,7 @@ Everything a pass needs is resident in HBM; a pass only uploads the per-bra
 prefix lengths and output offsets of each launch.
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -107,6 +108,7 @@ class ScreenedSet:
         self.basis = simint.Basis(shells)
         self.shells = shells
         self.out_doubles = out_doubles
+        self._fun_lock = threading.Lock()
         L = simint.lib()
         L.simint_init()
         # unique pairs by class
@@ -372,13 +374,18 @@ class ScreenedSet:
 
     def _group_fun(self, g):
         """Device array of the first basis function of both shells of every
-        pair of group g (built on first use)."""
-        if getattr(g, "dfun", None) is None:
-            import torch
-            f0 = np.r_[0, np.cumsum([simint.ncart(s[0]) for s in self.shells])].astype(np.int32)
-            fun = f0[np.asarray(g.pairs, dtype=np.int64).reshape(-1)]
-            g.dfun = torch.from_numpy(np.ascontiguousarray(fun, dtype=np.int32)).cuda()
-        return g.dfun
+        pair of group g (built once, on first use).  Built under a lock:
+        threads running tiles of one set side by side would otherwise each
+        build it, and replacing one thread's array frees it while that
+        thread's digestion still holds its address."""
+        with self._fun_lock:
+            if getattr(g, "dfun", None) is None:
+                import torch
+                f0 = np.r_[0, np.cumsum([simint.ncart(s[0]) for s in self.shells])].astype(np.int32)
+                fun = f0[np.asarray(g.pairs, dtype=np.int64).reshape(-1)]
+                g.dfun = torch.from_numpy(np.ascontiguousarray(fun, dtype=np.int32)).cuda()
+                torch.cuda.current_stream().synchronize()
+            return g.dfun
 
     def run_jk(self, out_ptr, d_density, d_J, d_K, launches=None, events=None):
         """Queue the launches (default: the whole pass) with the J/K

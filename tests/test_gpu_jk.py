@@ -146,3 +146,40 @@ def test_jk_mixed_plan(gpu):
     ej, ek = np.abs(J - Jr).max(), np.abs(K - Kr).max()
     print("mixed plan: max |dJ| %.2e, max |dK| %.2e" % (ej, ek))
     assert ej <= 1e-9 and ek <= 1e-9
+
+
+def test_group_function_arrays_shared_by_threads(gpu):
+    """Threads that start digesting one set side by side (the serving
+    workers) get one device function array per pair group, with the basis
+    function offsets of the group's pairs: an array built twice would be
+    freed under the first thread's digestion."""
+    import threading
+
+    import torch
+    shells = mol_shells("water.6-31gss")
+    ss = quartets.ScreenedSet(shells, tol=0.0)
+    try:
+        f0 = np.r_[0, np.cumsum([simint.ncart(s[0]) for s in shells])]
+        nthr = 8
+        go = threading.Barrier(nthr)
+        got = [None] * nthr
+
+        def run(t):
+            s = torch.cuda.Stream()
+            go.wait()
+            with torch.cuda.stream(s):
+                got[t] = [ss._group_fun(g) for g in (ss.groups if t % 2 else ss.groups[::-1])]
+
+        th = [threading.Thread(target=run, args=(t,)) for t in range(nthr)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for t in range(nthr):
+            arrs = got[t] if t % 2 else got[t][::-1]
+            for g, a in zip(ss.groups, arrs):
+                assert a is g.dfun
+                want = f0[np.asarray(g.pairs, dtype=np.int64).reshape(-1)]
+                assert np.array_equal(a.cpu().numpy(), want)
+    finally:
+        ss.free()
