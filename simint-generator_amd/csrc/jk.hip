@@ -135,17 +135,22 @@ __global__ void __launch_bounds__(256) jk_digest(const WorkItem *items, long lon
 // The same digestion with the ket shells' and the second bra shell's
 // component counts fixed at compile time: each lane reads every row
 // (a, b, :, :) of its quartet once and applies all six updates from
-// registers (the density values it needs are loaded once per item, or once
-// per a), instead of one pass over the quartet per update.  rows != nullptr:
-// the exchange updates whose row is a bra function (K_a., K_b.) go to that
-// block's row accumulator in the local data share -- row a of the first bra
-// shell at rows[a * nbf], row b of the second at rows[(na + b) * nbf] --
-// instead of global memory.
-template <int NB, int NC, int ND, bool ROWS>
+// registers (the density values it needs are loaded once per item, per a or
+// per row), instead of one pass over the quartet per update.  The second
+// ket shell's components are taken DC at a time (DC divides ND, chosen so
+// the accumulators stay in registers), each chunk reading its columns of
+// every row.  rows != nullptr (ROWS): the exchange updates whose row is a
+// bra function (K_a., K_b.) go to that block's row accumulator in the local
+// data share -- row a of the first bra shell at rows[a * nbf], row b of the
+// second at rows[(na + b) * nbf] -- and K_b. is added there after every
+// row (a, b); without it they go to global memory, K_b. summed over a first.
+template <int NB, int NC, int ND, int DC, bool ROWS>
 __device__ __forceinline__ void digest_item(const WorkItem &it, int lane, const long long *col, const double *ints,
                                             int na, const JkDev &jk, double *rows)
 {
+    static_assert(ND % DC == 0, "chunks of the second ket shell divide it");
     constexpr int NCD = NC * ND;
+    constexpr int KB = ROWS ? 1 : NB;   // K_b. accumulators held across a
     const int n = jk.nbf;
     const int fa = jk.bra_fun[2 * it.bra], fb = jk.bra_fun[2 * it.bra + 1];
     if (fa < 0 || fb < 0 || fa + na > n || fb + NB > n) return;
@@ -178,96 +183,122 @@ __device__ __forceinline__ void digest_item(const WorkItem &it, int lane, const 
     if (fa == fb) f *= 0.5;
     if (fc == fd) f *= 0.5;
     if (jk.same && k == it.bra) f *= 0.5;
-    double dcd[NCD], dbd[NB][ND], dbc[NB][NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++)
-#pragma unroll
-        for (int d = 0; d < ND; d++) dcd[c * ND + d] = Dl((long long)(fc + c) * n + fd + d);
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-#pragma unroll
-        for (int d = 0; d < ND; d++) dbd[b][d] = Dl((long long)(fb + b) * n + fd + d);
-#pragma unroll
-        for (int c = 0; c < NC; c++) dbc[b][c] = Dl((long long)(fb + b) * n + fc + c);
-    }
-    double jcd[NCD], kbc[NB][NC], kbd[NB][ND];
-#pragma unroll
-    for (int i = 0; i < NCD; i++) jcd[i] = 0.0;
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-#pragma unroll
-        for (int c = 0; c < NC; c++) kbc[b][c] = 0.0;
-#pragma unroll
-        for (int d = 0; d < ND; d++) kbd[b][d] = 0.0;
-    }
-    for (int a = 0; a < na; a++) {
-        double dad[ND], dac[NC], kac[NC], kad[ND];
-#pragma unroll
-        for (int d = 0; d < ND; d++) {
-            dad[d] = Dl((long long)(fa + a) * n + fd + d);
-            kad[d] = 0.0;
-        }
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-            dac[c] = Dl((long long)(fa + a) * n + fc + c);
-            kac[c] = 0.0;
-        }
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const double *r = q + (a * NB + b) * NCD;
-            const double dab = D[(long long)(fa + a) * n + fb + b];
-            double jab = 0.0;
-#pragma unroll
-            for (int c = 0; c < NC; c++)
-#pragma unroll
-                for (int d = 0; d < ND; d++) {
-                    const double v = act ? r[c * ND + d] : 0.0;   // r is readable either way
-                    jab = fma(v, dcd[c * ND + d], jab);
-                    jcd[c * ND + d] = fma(v, dab, jcd[c * ND + d]);
-                    kac[c] = fma(v, dbd[b][d], kac[c]);
-                    kad[d] = fma(v, dbc[b][c], kad[d]);
-                    kbc[b][c] = fma(v, dad[d], kbc[b][c]);
-                    kbd[b][d] = fma(v, dac[c], kbd[b][d]);
-                }
-            if (jk.J) {
-                jab = wave_sum(f * jab);
-                if (lane == 0) unsafeAtomicAdd(jk.J + (long long)(fa + a) * n + fb + b, 4.0 * jab);
-            }
-        }
-        if (jk.K && act) {
-#pragma unroll
-            for (int c = 0; c < NC; c++) addK(a, fc + c, 2.0 * f * kac[c]);
-#pragma unroll
-            for (int d = 0; d < ND; d++) addK(a, fd + d, 2.0 * f * kad[d]);
-        }
-    }
-    if (!act) return;
-    if (jk.J) {
+    const double g = 2.0 * f;
+    for (int d0 = 0; d0 < ND; d0 += DC) {
+        double dcd[NC * DC], jcd[NC * DC], kbc[KB][NC], kbd[KB][DC];
 #pragma unroll
         for (int c = 0; c < NC; c++)
 #pragma unroll
-            for (int d = 0; d < ND; d++)
-                unsafeAtomicAdd(jk.J + (long long)(fc + c) * n + fd + d, 4.0 * f * jcd[c * ND + d]);
-    }
-    if (jk.K) {
+            for (int d = 0; d < DC; d++) {
+                dcd[c * DC + d] = Dl((long long)(fc + c) * n + fd + d0 + d);
+                jcd[c * DC + d] = 0.0;
+            }
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
+        for (int b = 0; b < KB; b++) {
 #pragma unroll
-            for (int c = 0; c < NC; c++) addK(na + b, fc + c, 2.0 * f * kbc[b][c]);
+            for (int c = 0; c < NC; c++) kbc[b][c] = 0.0;
 #pragma unroll
-            for (int d = 0; d < ND; d++) addK(na + b, fd + d, 2.0 * f * kbd[b][d]);
+            for (int d = 0; d < DC; d++) kbd[b][d] = 0.0;
+        }
+        for (int a = 0; a < na; a++) {
+            double dad[DC], dac[NC], kac[NC], kad[DC];
+#pragma unroll
+            for (int d = 0; d < DC; d++) {
+                dad[d] = Dl((long long)(fa + a) * n + fd + d0 + d);
+                kad[d] = 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                dac[c] = Dl((long long)(fa + a) * n + fc + c);
+                kac[c] = 0.0;
+            }
+            // one row (a, b, :, d0..d0+DC) of the quartet
+            auto row = [&](int b) {
+                const int kb = ROWS ? 0 : b;
+                if constexpr (ROWS) {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) kbc[0][c] = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DC; d++) kbd[0][d] = 0.0;
+                }
+                double dbd[DC], dbc[NC];
+#pragma unroll
+                for (int d = 0; d < DC; d++) dbd[d] = Dl((long long)(fb + b) * n + fd + d0 + d);
+#pragma unroll
+                for (int c = 0; c < NC; c++) dbc[c] = Dl((long long)(fb + b) * n + fc + c);
+                const double *r = q + (a * NB + b) * NCD + d0;
+                const double dab = D[(long long)(fa + a) * n + fb + b];
+                double jab = 0.0;
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+#pragma unroll
+                    for (int d = 0; d < DC; d++) {
+                        const double v = act ? r[c * ND + d] : 0.0;   // r is readable either way
+                        jab = fma(v, dcd[c * DC + d], jab);
+                        jcd[c * DC + d] = fma(v, dab, jcd[c * DC + d]);
+                        kac[c] = fma(v, dbd[d], kac[c]);
+                        kad[d] = fma(v, dbc[c], kad[d]);
+                        kbc[kb][c] = fma(v, dad[d], kbc[kb][c]);
+                        kbd[kb][d] = fma(v, dac[c], kbd[kb][d]);
+                    }
+                if (jk.J) {
+                    jab = wave_sum(f * jab);
+                    if (lane == 0) unsafeAtomicAdd(jk.J + (long long)(fa + a) * n + fb + b, 4.0 * jab);
+                }
+                if constexpr (ROWS) {
+                    if (jk.K && act) {
+#pragma unroll
+                        for (int c = 0; c < NC; c++) addK(na + b, fc + c, g * kbc[0][c]);
+#pragma unroll
+                        for (int d = 0; d < DC; d++) addK(na + b, fd + d0 + d, g * kbd[0][d]);
+                    }
+                }
+            };
+            if constexpr (ROWS) {
+                // nothing is indexed by b: a loop, so one row's loads are in
+                // flight at a time
+#pragma unroll 1
+                for (int b = 0; b < NB; b++) row(b);
+            } else {
+#pragma unroll
+                for (int b = 0; b < NB; b++) row(b);
+            }
+            if (jk.K && act) {
+#pragma unroll
+                for (int c = 0; c < NC; c++) addK(a, fc + c, g * kac[c]);
+#pragma unroll
+                for (int d = 0; d < DC; d++) addK(a, fd + d0 + d, g * kad[d]);
+            }
+        }
+        if (act && jk.J) {
+#pragma unroll
+            for (int c = 0; c < NC; c++)
+#pragma unroll
+                for (int d = 0; d < DC; d++)
+                    unsafeAtomicAdd(jk.J + (long long)(fc + c) * n + fd + d0 + d, 4.0 * f * jcd[c * DC + d]);
+        }
+        if constexpr (!ROWS) {
+            if (act && jk.K) {
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+#pragma unroll
+                    for (int c = 0; c < NC; c++) addK(na + b, fc + c, g * kbc[b][c]);
+#pragma unroll
+                    for (int d = 0; d < DC; d++) addK(na + b, fd + d0 + d, g * kbd[b][d]);
+                }
+            }
         }
     }
 }
 
 // one wavefront per work item
-template <int NB, int NC, int ND>
+template <int NB, int NC, int ND, int DC>
 __global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long long nitems, const long long *col,
                                                    const double *ints, int na, JkDev jk)
 {
     const long long w = (long long)blockIdx.x * 4 + threadIdx.x / 64;
     if (w >= nitems) return;   // uniform over the wave
-    digest_item<NB, NC, ND, false>(items[w], threadIdx.x & 63, col, ints, na, jk, nullptr);
+    digest_item<NB, NC, ND, DC, false>(items[w], threadIdx.x & 63, col, ints, na, jk, nullptr);
 }
 
 // One block per bra pair of the launch: its four wavefronts take the bra's
@@ -275,8 +306,9 @@ __global__ void __launch_bounds__(256) jk_digest_t(const WorkItem *items, long l
 // summed over all its kets in the local data share ((na + NB) x nbf
 // doubles, dynamic) and added to K once per touched element at the end --
 // never more global atomics than one per quartet and element, usually far
-// fewer (every ket of the bra adds to the same rows).
-template <int NB, int NC, int ND>
+// fewer (every ket of the bra adds to the same rows).  Launched only with
+// an exchange matrix to fill.
+template <int NB, int NC, int ND, int DC>
 __global__ void __launch_bounds__(256) jk_digest_rows(const WorkItem *items, long long nitems,
                                                       const long long *tstart, int nbra, const long long *col,
                                                       const double *ints, int na, JkDev jk)
@@ -287,19 +319,11 @@ __global__ void __launch_bounds__(256) jk_digest_rows(const WorkItem *items, lon
     if (i0 >= i1) return;   // uniform over the block
     const int n = jk.nbf;
     const int R = na + NB;
-    const bool use_rows = jk.K != nullptr;
-    if (use_rows) {
-        for (long long e = threadIdx.x; e < (long long)R * n; e += 256) rows[e] = 0.0;
-        __syncthreads();
-    }
+    for (long long e = threadIdx.x; e < (long long)R * n; e += 256) rows[e] = 0.0;
+    __syncthreads();
     const int lane = threadIdx.x & 63;
-    if (!use_rows) {
-        for (long long w = i0 + threadIdx.x / 64; w < i1; w += 4)
-            digest_item<NB, NC, ND, false>(items[w], lane, col, ints, na, jk, nullptr);
-        return;
-    }
     for (long long w = i0 + threadIdx.x / 64; w < i1; w += 4)
-        digest_item<NB, NC, ND, true>(items[w], lane, col, ints, na, jk, rows);
+        digest_item<NB, NC, ND, DC, true>(items[w], lane, col, ints, na, jk, rows);
     __syncthreads();
     const int bra = items[i0].bra;
     const int fa = jk.bra_fun[2 * bra], fb = jk.bra_fun[2 * bra + 1];
@@ -314,9 +338,26 @@ __global__ void __launch_bounds__(256) jk_digest_rows(const WorkItem *items, lon
 
 constexpr int nc_of(int l) { return (l + 1) * (l + 2) / 2; }
 
-// register arrays of the specialised kernel (doubles per lane); larger
-// shapes take the generic kernel
-constexpr bool fits_t(int nb, int nc, int nd) { return 3 * nc * nd + 2 * nb * (nc + nd) + 2 * (nc + nd) <= 192; }
+// doubles per lane the specialised kernel holds for a chunk of dc
+// components of the second ket shell: density and Coulomb sums of the ket
+// block, the per-a and per-row vectors, and without the row accumulator the
+// K_b. sums over a plus (a quarter of) the unrolled rows' loads in flight.
+// Kept within kRegDoubles (gfx950 has 256 VGPRs per lane; the rest hold
+// addresses and loads): every instance then runs without scratch.
+constexpr int kRegDoubles = 96;
+constexpr int reg_doubles(int nb, int nc, int dc, bool rows)
+{
+    return 2 * nc * dc + 4 * (nc + dc) + (rows ? 0 : nb * (nc + dc) + nb * nc * dc / 4);
+}
+
+// components of the second ket shell per chunk: the largest divisor of nd
+// that fits; 0: none does (the generic kernel)
+constexpr int pick_dc(int nb, int nc, int nd, bool rows)
+{
+    for (int dc = nd; dc >= 1; --dc)
+        if (nd % dc == 0 && reg_doubles(nb, nc, dc, rows) <= kRegDoubles) return dc;
+    return 0;
+}
 
 // exchange row accumulators of the per-bra kernel: at most this many bytes
 // of the local data share per block (and no more than the device allows)
@@ -354,22 +395,31 @@ hipError_t launch_t(const Launch &L, const JkDev &d, hipStream_t s)
 {
     constexpr int NB = nc_of(LB), NC = nc_of(LC), ND = nc_of(LD);
     const long long rbytes = 8LL * (L.na + NB) * d.nbf;
-    if constexpr (fits_t(NB, NC, ND)) {
-        if (d.K && rbytes <= row_bytes_limit())
-            hipLaunchKernelGGL((jk_digest_rows<NB, NC, ND>), dim3((unsigned)L.nbra), dim3(256), (size_t)rbytes, s,
-                               L.items, L.nitems, L.tstart, L.nbra, L.col, L.ints, L.na, d);
-        else
-            hipLaunchKernelGGL((jk_digest_t<NB, NC, ND>), dim3((unsigned)((L.nitems + 3) / 4)), dim3(256), 0, s,
-                               L.items, L.nitems, L.col, L.ints, L.na, d);
-    } else {
-        const WorkItem *items = L.items;
-        const long long nitems = L.nitems;
-        const long long *col = L.col;
-        const double *ints = L.ints;
-        const int na = L.na;
-        hipLaunchKernelGGL(jk_digest, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, items, nitems, col, ints,
-                           na, NB, NC, ND, d);
+    constexpr int DCR = pick_dc(NB, NC, ND, true), DCG = pick_dc(NB, NC, ND, false);
+    const WorkItem *items = L.items;
+    const long long nitems = L.nitems;
+    const long long *col = L.col;
+    const double *ints = L.ints;
+    const int na = L.na;
+    const unsigned nwave_blocks = (unsigned)((nitems + 3) / 4);
+    bool done = false;
+    if constexpr (DCR > 0) {
+        if (d.K && rbytes <= row_bytes_limit()) {
+            hipLaunchKernelGGL((jk_digest_rows<NB, NC, ND, DCR>), dim3((unsigned)L.nbra), dim3(256), (size_t)rbytes,
+                               s, items, nitems, L.tstart, L.nbra, col, ints, na, d);
+            done = true;
+        }
     }
+    if constexpr (DCG > 0) {
+        if (!done) {
+            hipLaunchKernelGGL((jk_digest_t<NB, NC, ND, DCG>), dim3(nwave_blocks), dim3(256), 0, s, items, nitems,
+                               col, ints, na, d);
+            done = true;
+        }
+    }
+    if (!done)
+        hipLaunchKernelGGL(jk_digest, dim3(nwave_blocks), dim3(256), 0, s, items, nitems, col, ints, na, NB, NC, ND,
+                           d);
     return hipGetLastError();
 }
 
