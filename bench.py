@@ -36,8 +36,10 @@ Besides the metric the line carries:
   batched           1e6 quartets of all 256 classes up to f in one graph;
   deriv1            first derivatives of the 64-atom system;
   fp64_64atom       the 64-atom system in FP64 (the all-classes-to-d config),
-                    and the same pass with the Coulomb/exchange digestion
-                    after every launch (fp64_64atom.jk);
+                    the same pass with the Coulomb/exchange digestion
+                    after every launch (fp64_64atom.jk), and with the
+                    primitive Schwarz screen at 1e-13 (fp64_64atom.prim_screen,
+                    with its largest change over every integral);
   dropin_pcie       the host-buffer drop-in call on one large batch (its
                     rate includes the PCIe copies, which `value` does not);
   cpu_baseline      the CPU oracle (a C restatement of the reference
@@ -568,7 +570,44 @@ def fp64_64atom_leg():
                  "pass_s": wjk, "value": ss.total_quartets / wjk, "unit": "shell quartets/s",
                  "digestion_share": 1.0 - wall / wjk,
                  "energy_check": {"tr_DJ": float((D * J).sum()), "tr_DK": float((D * K).sum())}}
+    del J, K, D
+    res["prim_screen"] = prim_screen_leg(ss, out)
     ss.free()
+    return res
+
+
+def prim_screen_leg(ss, out, prim_tol=1e-13):
+    """The same 64-atom pass with simint's primitive screen: pairs built
+    with SIMINT_SCREEN_SCHWARZ (a bound per primitive pair) and screen_tol
+    applied in the primitive loop; the change against the unscreened pass
+    is measured over every integral."""
+    import torch
+    import quartets
+    import simint
+    import workloads
+    s2 = quartets.ScreenedSet(workloads.grid_molecule(natom=64, seed=0), tol=1e-12, prim_tol=prim_tol)
+    if [(ln.cls, ln.b0, ln.nq) for ln in s2.launches] != [(ln.cls, ln.b0, ln.nq) for ln in ss.launches]:
+        raise RuntimeError("the screened set's plan differs from the plain one")
+    out2 = torch.empty(s2.out_doubles, dtype=torch.float64, device="cuda")
+    s2.run_launches(out2.data_ptr(), shares(s2.launches, 4)[0])
+    simint.synchronize()
+    t = time.perf_counter()
+    s2.run_pass(out2.data_ptr())
+    simint.synchronize()
+    wall = time.perf_counter() - t
+    change = 0.0
+    for c in range(ss.nchunk):
+        ss.run_pass(out.data_ptr(), chunk=c)
+        s2.run_pass(out2.data_ptr(), chunk=c)
+        simint.synchronize()
+        n = ss.chunk_doubles(c)
+        change = max(change, float((out[:n] - out2[:n]).abs().max()))
+    res = {"what": "the 64-atom FP64 pass with the primitive Schwarz screen (SIMINT_SCREEN_SCHWARZ pairs, "
+                   "screen_tol %g in the primitive loop)" % prim_tol,
+           "value": s2.total_quartets / wall, "unit": "shell quartets/s", "pass_s": wall,
+           "max_abs_change_vs_unscreened": change, "setup_s": s2.setup_s}
+    del out2
+    s2.free()
     return res
 
 

@@ -42,14 +42,14 @@ class Group:
     With a device shell set the pairs are filled on the device and the
     bounds stay there too (dbounds)."""
 
-    def __init__(self, basis, cls, nprim, pairs, bounds, dshells=None, dbounds=None):
+    def __init__(self, basis, cls, nprim, pairs, bounds, dshells=None, dbounds=None, screen=simint.SCREEN_NONE):
         self.cls = cls
         self.nprim = nprim
         self.pairs = pairs
         self.bounds = bounds
         self.dbounds = dbounds
         if dshells is None:
-            self.host = simint.Pairs.from_pairs(basis, pairs)
+            self.host = simint.Pairs.from_pairs(basis, pairs, screen)
             self.dev = simint.DevicePairs(self.host)
         else:
             self.host = None
@@ -91,13 +91,19 @@ class ScreenedSet:
     mixed: base threshold (see mixed_threshold) below which a quartet's
     bound sends it to the FP32-VRR kernels (None: every quartet FP64)."""
 
-    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None, deriv=0, front_end="host"):
+    def __init__(self, shells, tol=1e-12, maxam=2, out_doubles=1 << 29, mixed=None, deriv=0, front_end="host",
+                 prim_tol=0.0):
         """front_end "host": shell pairs filled on the host (the drop-in
         simint_create_multi_shellpair2) and uploaded, bounds sorted and
         screened quartets counted in numpy; "device": pairs filled, bounds
         computed and sorted, and counts taken on the GPU from a device copy
         of the shells (only the shell-index lists go up and the per-bra
-        counts come back)."""
+        counts come back).
+        prim_tol > 0: the pairs carry Schwarz primitive bounds
+        (SIMINT_SCREEN_SCHWARZ) and the kernels skip every primitive
+        quartet whose bound product is below prim_tol (simint's screen_tol
+        in the primitive loop); each skipped one moves an integral by at
+        most prim_tol.  Host front end and ERIs only."""
         import time
         import torch
         t0 = time.perf_counter()
@@ -105,6 +111,10 @@ class ScreenedSet:
         self.mixed = mixed
         self.deriv = deriv
         self.front_end = front_end
+        if prim_tol < 0 or (prim_tol > 0 and (front_end != "host" or deriv)):
+            raise ValueError("prim_tol: a tolerance >= 0, with the host front end and plain ERIs")
+        self.prim_tol = float(prim_tol)
+        screen = simint.SCREEN_SCHWARZ if prim_tol > 0 else simint.SCREEN_NONE
         self.basis = simint.Basis(shells)
         self.shells = shells
         self.out_doubles = out_doubles
@@ -152,7 +162,8 @@ class ScreenedSet:
             for np_ in sorted(set(nprim.tolist()), reverse=True):
                 sel = np.nonzero(nprim == np_)[0]
                 order = sel[np.argsort(-diag[sel], kind="stable")]
-                self.groups.append(Group(self.basis, cls, int(np_), [pairs[k] for k in order], diag[order].copy()))
+                self.groups.append(Group(self.basis, cls, int(np_), [pairs[k] for k in order], diag[order].copy(),
+                                         screen=screen))
         import json
         import os
         here = os.path.dirname(os.path.abspath(__file__))
@@ -310,9 +321,9 @@ class ScreenedSet:
             if self.deriv:
                 simint.compute_deriv1_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, self.work_ptr, out_ptr)
             elif ln.lo is None and ln.precision == 0:
-                simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out_ptr, 0.0)
+                simint.compute_prefix(ln.bra.dev, ln.ket.dev, ln.b0, ln.nket, ln.off, out_ptr, self.prim_tol)
             else:
-                simint.compute_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, 0.0,
+                simint.compute_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, self.prim_tol,
                                      ln.precision)
             if events:
                 events(ln, False)
@@ -400,7 +411,7 @@ class ScreenedSet:
                 events(ln, True)
             jk = simint.JK(nbf, 1 if ln.bra is ln.ket else 0, self._group_fun(ln.bra).data_ptr(),
                            self._group_fun(ln.ket).data_ptr(), d_density, d_J, d_K)
-            simint.compute_jk_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, jk, 0.0,
+            simint.compute_jk_range(ln.bra.dev, ln.ket.dev, ln.b0, ln.lo, ln.nket, ln.off, out_ptr, jk, self.prim_tol,
                                     ln.precision)
             if events:
                 events(ln, False)

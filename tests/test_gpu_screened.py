@@ -6,7 +6,12 @@ use, against the oracle:
     sample of its quartets located in the output chunks;
   * the drop-in simint_compute_eri called from several host threads at once
     on a shared ket pair set (the reference's tests do this with OpenMP,
-    skel/test/test_eri.cpp:120-134): every thread's results match the oracle.
+    skel/test/test_eri.cpp:120-134): every thread's results match the oracle;
+  * the same pass with the primitive Schwarz screen (pairs built with
+    SIMINT_SCREEN_SCHWARZ, screen_tol in the primitive loop): every
+    integral within K * tol of the unscreened pass (K its primitive
+    quartets; each skipped one has a bound below tol), and something is
+    skipped at a loose tolerance.
 """
 import threading
 
@@ -97,3 +102,36 @@ def test_dropin_concurrent_host_threads(gpu):
     Q.free()
     basis.free()
     assert not errors, errors
+
+
+@pytest.mark.parametrize("tol", [1e-12, 1e-9])
+def test_primitive_schwarz_screen_within_bound(gpu, tol):
+    import torch
+    shells = workloads.grid_molecule(natom=6, seed=7)
+    plain = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24)
+    scr = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 24, prim_tol=tol)
+    try:
+        sig = [(ln.cls, ln.b0, ln.nq, ln.chunk) for ln in plain.launches]
+        assert sig == [(ln.cls, ln.b0, ln.nq, ln.chunk) for ln in scr.launches]
+        a = torch.empty(plain.out_doubles, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        worst, moved = 0.0, 0.0
+        for c in range(plain.nchunk):
+            plain.run_pass(a.data_ptr(), chunk=c)
+            scr.run_pass(b.data_ptr(), chunk=c)
+            simint.synchronize()
+            for lp, ls in zip(plain.chunk_launches(c), scr.chunk_launches(c)):
+                o = int(lp.off[0])
+                x, y = a[o:o + lp.nout], b[o:o + lp.nout]
+                K = lp.bra.nprim * lp.ket.nprim
+                d = float((x - y).abs().max())
+                bound = K * tol + 1e-14 * max(1.0, float(x.abs().max()))
+                worst = max(worst, d / bound)
+                moved = max(moved, d)
+        print("prim_tol %g: max change %.3e, worst change / bound %.3f" % (tol, moved, worst))
+        assert worst <= 1.0
+        if tol >= 1e-9:
+            assert moved > 1e-12   # the screen skipped primitive quartets
+    finally:
+        plain.free()
+        scr.free()
