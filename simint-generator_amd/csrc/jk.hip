@@ -360,8 +360,10 @@ constexpr int pick_dc(int nb, int nc, int nd, bool rows)
 }
 
 // exchange row accumulators of the per-bra kernel: at most this many bytes
-// of the local data share per block (and no more than the device allows)
-constexpr long long kRowBytes = 96 * 1024;
+// of the local data share per block (and no more than the device allows,
+// 160 KB on gfx950).  A block above 80 KB holds its CU's share alone
+// whatever its size, so the whole of it is offered.
+constexpr long long kRowBytes = 160 * 1024;
 
 std::atomic<int> g_rows_enabled{1};   // simint_hip_set_jk_rows
 

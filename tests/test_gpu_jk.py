@@ -183,3 +183,37 @@ def test_group_function_arrays_shared_by_threads(gpu):
                 assert np.array_equal(a.cpu().numpy(), want)
     finally:
         ss.free()
+
+
+def test_jk_rows_beyond_64k_of_lds(gpu):
+    """An 80-atom system (1200 basis functions): a (dd| bra's twelve exchange
+    rows take 115 KB of the local data share, more than 64 KB and within
+    the device's 160 KB; J and K with the rows match those with per-quartet
+    atomics (simint_hip_set_jk_rows(0)) to rounding."""
+    import torch
+    shells = workloads.grid_molecule(natom=80, seed=11)
+    ss = quartets.ScreenedSet(shells, tol=1e-12, out_doubles=1 << 26)
+    L = simint.lib()
+    prev = L.simint_hip_set_jk_rows(1)
+    try:
+        nbf = ss.nbf()
+        assert 12 * nbf * 8 > 96 * 1024
+        D = torch.from_numpy(_density(nbf, 11)).cuda()
+        out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+        res = []
+        for rows in (1, 0):
+            L.simint_hip_set_jk_rows(rows)
+            J = torch.zeros((nbf, nbf), dtype=torch.float64, device="cuda")
+            K = torch.zeros_like(J)
+            ss.run_jk(out.data_ptr(), D.data_ptr(), J.data_ptr(), K.data_ptr())
+            simint.jk_symmetrize(nbf, J.data_ptr(), K.data_ptr())
+            simint.synchronize()
+            res.append((J.cpu().numpy(), K.cpu().numpy()))
+        (J1, K1), (J0, K0) = res
+        ej = np.abs(J1 - J0).max() / np.abs(J0).max()
+        ek = np.abs(K1 - K0).max() / np.abs(K0).max()
+        print("nbf %d: rows vs per-quartet atomics, J %.2e K %.2e normwise" % (nbf, ej, ek))
+        assert ej <= 1e-13 and ek <= 1e-13
+    finally:
+        L.simint_hip_set_jk_rows(prev)
+        ss.free()
