@@ -473,13 +473,17 @@ def batched_leg(steps, warmup, nquartet=10 ** 6):
 
 def deriv_leg(natom=64, tol=1e-12):
     """First derivatives of every screened quartet of the 64-atom system
-    (12 values per Cartesian), one pass through a 1 GiB chunk buffer."""
+    (12 values per Cartesian), one pass through an 8 GiB chunk buffer: each
+    derivative call runs a class kernel per (a+1)/(a-1) term and the
+    assembly, so a call carries ~0.5 ms of launch ramp-up and tail, and
+    larger chunks (fewer calls) amortise it (11.2 s per pass with 1 GiB
+    chunks, 8.8 s with 8 GiB; the term scratch grows to 6 GB)."""
     import torch
     import quartets
     import simint
     import workloads
     shells = workloads.grid_molecule(natom=natom, seed=0)
-    ss = quartets.ScreenedSet(shells, tol=tol, out_doubles=1 << 27, deriv=1)
+    ss = quartets.ScreenedSet(shells, tol=tol, out_doubles=1 << 30, deriv=1)
     work = torch.empty(ss.deriv_worksize(), dtype=torch.float64, device="cuda")
     out = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
     stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
