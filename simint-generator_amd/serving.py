@@ -145,7 +145,7 @@ def footprint(nshell, kind, chunk_doubles):
 
 class Server:
     def __init__(self, hbm_budget=64 << 30, slots=4, tol=1e-10, chunk_doubles=1 << 27, verbose=False, probe=0,
-                 timeout=None):
+                 timeout=None, deriv_chunk_doubles=None):
         self.verbose = verbose
         self.last_note = 0.0
         self.probe = probe         # quartets per request whose values are kept (result checks)
@@ -154,6 +154,10 @@ class Server:
         self.slots = slots
         self.tol = tol
         self.chunk = chunk_doubles
+        # derivative tiles are larger: every derivative call runs a class
+        # kernel per (a+1)/(a-1) term and the assembly, whose launch ramps
+        # and tails larger tiles amortise
+        self.deriv_chunk = deriv_chunk_doubles or 4 * chunk_doubles
         self.timeout = timeout     # default per-request time limit (s after admission)
         self.cv = threading.Condition()
         self.waiting = []          # arrived, not admitted (arrival order)
@@ -235,7 +239,8 @@ class Server:
             r.error = "rejected: " + bad
             return
         deriv = 1 if r.kind == "deriv" else 0
-        ss = quartets.ScreenedSet(shells, tol=self.tol, out_doubles=self.chunk, deriv=deriv)
+        ss = quartets.ScreenedSet(shells, tol=self.tol, out_doubles=self.deriv_chunk if deriv else self.chunk,
+                                  deriv=deriv)
         r.ss = ss
         r.quartets = ss.total_quartets
         r.left = float(ss.total_bytes)
@@ -262,6 +267,11 @@ class Server:
         import torch
         ss = r.ss
         n = ss.chunk_doubles(c)
+        if out[0].numel() < ss.out_doubles:
+            # this worker's earlier tiles ended with a stream synchronisation
+            out[0] = None
+            out[0] = torch.empty(ss.out_doubles, dtype=torch.float64, device="cuda")
+        out = out[0]
         wp = None
         if ss.deriv:
             need = r.work_need
@@ -284,7 +294,7 @@ class Server:
     def _worker(self, done):
         import torch
         stream = torch.cuda.ExternalStream(simint.lib().simint_hip_stream())
-        out = torch.empty(self.chunk, dtype=torch.float64, device="cuda")
+        out = [torch.empty(self.chunk, dtype=torch.float64, device="cuda")]   # grows for derivative tiles
         work = [None]
         while True:
             with self.cv:
