@@ -1,7 +1,9 @@
 """The screened-set launch planner (CPU): every screened quartet planned
 exactly once, every launch's output inside the chunk buffer -- including a
 bra whose own kets exceed the buffer (cut into ket ranges) -- and the
-mixed-precision split covering each range without overlap."""
+mixed-precision split covering each range without overlap; the options
+of the primitive screen and of serving's derivative tiles checked without a
+device."""
 import json
 import os
 
@@ -68,3 +70,21 @@ def test_plan_rejects_a_quartet_larger_than_the_buffer():
     groups = [_Group((2, 2), 1, [1.0, 0.5])]
     with pytest.raises(RuntimeError):
         _planned(groups, 1000, deriv=1)
+
+
+@pytest.mark.parametrize("kw", [dict(prim_tol=-1e-12), dict(prim_tol=1e-12, front_end="device"),
+                                dict(prim_tol=1e-12, deriv=1)])
+def test_primitive_screen_options_rejected_before_any_device_work(kw):
+    """The primitive Schwarz screen takes a tolerance >= 0, the host front
+    end (its pairs carry SIMINT_SCREEN_SCHWARZ bounds) and plain ERIs; other
+    combinations are refused before the set touches the library."""
+    with pytest.raises(ValueError, match="prim_tol"):
+        quartets.ScreenedSet([(0, (0.0, 0.0, 0.0), [1.0], [1.0])], **kw)
+
+
+def test_serving_derivative_tiles_are_larger():
+    """Derivative requests are cut into tiles four times the ERI tile unless
+    a size is given (their calls carry per-term launch costs)."""
+    import serving
+    assert serving.Server(chunk_doubles=1 << 20).deriv_chunk == 4 << 20
+    assert serving.Server(chunk_doubles=1 << 20, deriv_chunk_doubles=1 << 21).deriv_chunk == 1 << 21
