@@ -88,3 +88,12 @@ def test_serving_derivative_tiles_are_larger():
     import serving
     assert serving.Server(chunk_doubles=1 << 20).deriv_chunk == 4 << 20
     assert serving.Server(chunk_doubles=1 << 20, deriv_chunk_doubles=1 << 21).deriv_chunk == 1 << 21
+
+
+@pytest.mark.parametrize("cls,div", [((0, 0, 0, 0), 1), ((1, 0, 0, 0), 1), ((2, 0, 2, 0), 1),
+                                     ((1, 1, 0, 0), 400), ((2, 0, 1, 1), 400), ((2, 1, 2, 0), 400),
+                                     ((1, 1, 1, 1), 1e5), ((2, 2, 2, 1), 1e5)])
+def test_mixed_threshold_by_hrr_sides(cls, div):
+    """The FP32-VRR threshold: the base for classes without HRR, /400 with
+    one HRR side (both shells of a side above s), /1e5 with both."""
+    assert quartets.mixed_threshold(cls, 1e-4) == pytest.approx(1e-4 / div, rel=1e-15)
