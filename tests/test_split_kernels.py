@@ -3,7 +3,9 @@ their source text: every contracted input is accumulated exactly once per
 primitive quartet, and the ket-top split kernels' local-data-share use (the
 per-primitive-quartet setup cache, the output rows and their offsets) stays
 within one wavefront's share of a CU, with every cache access inside the
-declared array and guarded by the same slot count on both sides."""
+declared array and guarded by the same slot count on both sides; the
+refined reciprocal square root of p+q is used only where no HRR transfer
+can amplify its few ulp."""
 import re
 
 import pytest
@@ -57,3 +59,18 @@ def test_ket_top_split_kernel_lds_and_cache(am, G):
     assert sorted(int(v) for v in writes) == list(range(NV))          # group 0 writes every value once
     guards = re.findall(r"if \(t < (\d+)\)", src)
     assert guards and all(int(k) == KC for k in guards) and len(guards) == G   # group 0 + G-1 readers
+
+
+@pytest.mark.parametrize("am", [am for am in generate.classes(2)])
+def test_fast_reciprocal_square_root_only_without_hrr(am):
+    """1/(p+q) and sqrt(1/(p+q)) come from one refined reciprocal square root
+    (a few ulp) only in classes without an HRR transfer: an HRR transfer
+    (a,b+1| = (a+1,b| + AB (a,b| cancels strongly for far-apart pairs and
+    would amplify those ulp; classes with HRR divide and take the square
+    root exactly."""
+    g = graph(*am)
+    src = hip_lpq.emit_lpq(am, mixed=True)
+    fast = "rsqrt_nr(PQalpha_sum)" in src
+    assert fast == (not g.bra_hrr_needed() and not g.ket_hrr_needed()), am
+    if not fast:
+        assert "1.0 / PQalpha_sum" in src
