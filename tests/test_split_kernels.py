@@ -74,3 +74,26 @@ def test_fast_reciprocal_square_root_only_without_hrr(am):
     assert fast == (not g.bra_hrr_needed() and not g.ket_hrr_needed()), am
     if not fast:
         assert "1.0 / PQalpha_sum" in src
+
+
+def _split_all():
+    """Every split kernel the build emits for classes up to d: the default
+    kind's group counts and the ket-top variants."""
+    out = []
+    for am in generate.classes(2):
+        for G in generate.lpqs_variants(am):
+            out.append((am, G, None))
+        for G in generate.lpqsf_variants(am):
+            out.append((am, G, "f"))
+    return out
+
+
+@pytest.mark.parametrize("am,G,kind", _split_all())
+def test_split_groups_evaluate_boys_at_the_class_total(am, G, kind):
+    """Every group of a split kernel evaluates F_m at the class's total AM,
+    as one evaluation per primitive quartet does: with per-group orders the
+    long-range branch's error pattern differs between groups, and the HRR
+    transfers between tops of different groups no longer cancel it."""
+    src = hip_lpqs.emit_lpqs(am, G, kind)
+    orders = [int(x) for x in re.findall(r"boys_eval(?:_rs)?<(\d+)>\(grid", src)]
+    assert orders and set(orders) == {graph(*am).Ltot}, (am, G, kind, sorted(set(orders)))
