@@ -5,7 +5,9 @@ per-primitive-quartet setup cache, the output rows and their offsets) stays
 within one wavefront's share of a CU, with every cache access inside the
 declared array and guarded by the same slot count on both sides; the
 refined reciprocal square root of p+q is used only where no HRR transfer
-can amplify its few ulp."""
+can amplify its few ulp; every group of a split kernel evaluates the Boys
+function at the class's total AM; and the FP32-VRR kernels round only the
+VRR (setup, Boys, accumulation, HRR and stores stay FP64)."""
 import re
 
 import pytest
@@ -97,3 +99,34 @@ def test_split_groups_evaluate_boys_at_the_class_total(am, G, kind):
     src = hip_lpqs.emit_lpqs(am, G, kind)
     orders = [int(x) for x in re.findall(r"boys_eval(?:_rs)?<(\d+)>\(grid", src)]
     assert orders and set(orders) == {graph(*am).Ltot}, (am, G, kind, sorted(set(orders)))
+
+
+def _kernel_text(src, name):
+    i = src.index("__global__ void __launch_bounds__(64) %s(" % name)
+    j = src.find("\n}\n", i)
+    return src[i:j + 2]
+
+
+@pytest.mark.parametrize("am", generate.classes(2))
+def test_mixed_kernels_keep_setup_boys_accumulation_and_hrr_fp64(am):
+    """The FP32-VRR kernels round only the VRR: the Gaussian-product setup,
+    the Boys function and the prefactor are FP64, the VRR results are
+    accumulated into FP64 accumulators, and everything after the primitive
+    loops (the HRR transfers and the stores) is FP64 -- FP32 there lost
+    5e-7 on the 64-atom shells."""
+    name = "eri_lpqm_" + "_".join("spdf"[x] for x in am)
+    src = hip_lpq.emit_lpq(am, mixed=True)
+    if name + "(" not in src:
+        pytest.skip("no FP32-VRR variant for this class")
+    k = _kernel_text(src, name)
+    loop = k.index("    for (int i = ib0; i < ib1; ++i) {")
+    end = k.index("\n    }\n", loop)
+    setup, body, after = k[:loop], k[loop:end], k[end:]
+    assert "float" not in after and "fmaf" not in after
+    assert re.search(r"\bdouble acc0 = 0\.0;", setup)
+    assert not re.search(r"\bfloat acc\d+", k)
+    for v in ("PQalpha_sum", "one_over_PQalpha_sum", "F_x", "prefac"):
+        assert not re.search(r"\bfloat %s\b" % v, body), v
+    assert "boys_eval" in body                          # FP64 Boys inside the loops
+    if graph(*am).Ltot > 0:
+        assert "fmaf(" in body                          # and the FP32 VRR
