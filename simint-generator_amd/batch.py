@@ -105,10 +105,24 @@ class QuartetBatch:
                 simint.compute_deriv1_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(),
                                            self.work.data_ptr(), d_out, self.screen_tol)
             else:
-                simint.compute_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(), d_out,
-                                    self.screen_tol)
+                self._list(g, d_out)
             if events:
                 events(g, False)
+
+    def _list(self, g, d_out):
+        """One class's list launch, with its tuned lanes per quartet (if
+        any) for this launch only."""
+        lanes = simint.list_split_of(g.cls)
+        L = simint.lib()
+        prev = L.simint_hip_set_list_split(lanes) if lanes else None
+        if prev is not None and prev < 0:
+            raise ValueError("tuned list split of %s: %d lanes not accepted" % (g.cls, lanes))
+        try:
+            simint.compute_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(), d_out,
+                                self.screen_tol)
+        finally:
+            if prev is not None:
+                L.simint_hip_set_list_split(prev)
 
     def _prepare(self, n):
         """Side streams and per-stream derivative scratch, allocated before
@@ -138,8 +152,7 @@ class QuartetBatch:
                                                g.qout.data_ptr(), self._works[k % n].data_ptr(), d_out,
                                                self.screen_tol)
                 else:
-                    simint.compute_list(g.P, g.Q, g.n, g.idx.data_ptr(), g.idx.data_ptr(), g.qout.data_ptr(),
-                                        d_out, self.screen_tol)
+                    self._list(g, d_out)
         finally:
             L.simint_hip_set_stream(ctypes.c_void_p(prev))
         for s in self._side:

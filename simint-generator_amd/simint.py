@@ -332,10 +332,28 @@ class DevicePairs:
 FAMILIES = {"lpq": 1, "wpq": 2, "lpqs_a": 3, "lpqs_b": 4, "lpqsf_a": 5, "lpqsf_b": 6}
 
 
+_LIST_SPLIT = None   # class name -> lanes per quartet of its list launches
+
+
+def list_split_of(cls):
+    """Lanes per quartet a batch gives the lane-per-quartet list launches of
+    class cls: the tuned value ("list_split" of the package's tuning.json,
+    or of the last file passed to apply_tuning), 0 = the library's
+    automatic choice (about 65536 lanes per launch)."""
+    global _LIST_SPLIT
+    if _LIST_SPLIT is None:
+        import json
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+        _LIST_SPLIT = dict(json.load(open(path)).get("list_split", {})) if os.path.exists(path) else {}
+    return int(_LIST_SPLIT.get("_".join("spdfghik"[x] for x in cls), 0))
+
+
 def apply_tuning(path):
-    """Set each class's kernel family from a tuning file (as autotune.py
-    writes it) at run time; returns the number of classes changed."""
+    """Set each class's kernel family (and the list launches' lanes per
+    quartet) from a tuning file (as autotune.py writes it) at run time;
+    returns the number of classes changed."""
     import json
+    global _LIST_SPLIT
     n = 0
     d = json.load(open(path))
     for name, fam in d.get("family", {}).items():
@@ -346,6 +364,12 @@ def apply_tuning(path):
         am = ["spdfghik".index(c) for c in name.split("_")]
         if lib().simint_hip_set_class_list_kernel(*am, FAMILIES[fam]) >= 0:
             n += 1
+    if "list_split" in d:
+        for name, lanes in d["list_split"].items():
+            if lanes not in (0, 1, 2, 4, 8, 16):
+                raise ValueError("list_split %s: %r lanes (0, 1, 2, 4, 8 or 16)" % (name, lanes))
+        _LIST_SPLIT = dict(d["list_split"])
+        n += len(_LIST_SPLIT)
     return n
 
 

@@ -97,3 +97,27 @@ def test_mixed_threshold_by_hrr_sides(cls, div):
     """The FP32-VRR threshold: the base for classes without HRR, /400 with
     one HRR side (both shells of a side above s), /1e5 with both."""
     assert quartets.mixed_threshold(cls, 1e-4) == pytest.approx(1e-4 / div, rel=1e-15)
+
+
+def test_list_split_tuning_applied_per_class(tmp_path):
+    """Lanes per quartet of a class's list launches: 0 (the library's
+    automatic choice) unless a tuning file names a count, which
+    apply_tuning takes (1, 2, 4, 8 or 16 only)."""
+    import json
+
+    import simint
+    saved = simint._LIST_SPLIT
+    try:
+        simint._LIST_SPLIT = None
+        pkg = json.load(open(os.path.join(ROOT, "simint-generator_amd", "tuning.json")))
+        assert simint.list_split_of((1, 0, 0, 0)) == pkg.get("list_split", {}).get("p_s_s_s", 0)
+        f = tmp_path / "t.json"
+        f.write_text(json.dumps({"list_split": {"p_s_s_s": 4, "d_d_s_s": 16}}))
+        assert simint.apply_tuning(str(f)) == 2
+        assert simint.list_split_of((1, 0, 0, 0)) == 4 and simint.list_split_of((2, 2, 0, 0)) == 16
+        assert simint.list_split_of((1, 1, 1, 1)) == 0
+        f.write_text(json.dumps({"list_split": {"p_s_s_s": 3}}))
+        with pytest.raises(ValueError, match="lanes"):
+            simint.apply_tuning(str(f))
+    finally:
+        simint._LIST_SPLIT = saved
