@@ -615,7 +615,9 @@ def prim_screen_leg(ss, out, prim_tol=1e-13):
     return res
 
 
-def main():
+def make_parser():
+    """The command line (also parsed by the CPU tests against the commands
+    the profile summaries give)."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="shares of the quartet set timed (together one pass)")
@@ -638,7 +640,11 @@ def main():
                     help="profiling aid: symmetrise one N x N FP64 matrix with the library's kernel "
                          "(exactly N(N-1) FP64 flops: one add and one multiply per off-diagonal pair) "
                          "and exit (calibrates the SQ_INSTS_VALU_FLOPS_FP64 counter)")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = make_parser().parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (shell quartets are independent; a "
               "multi-GPU run would shard the screened set by bra pairs with replicated shell pairs)",

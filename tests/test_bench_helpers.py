@@ -60,3 +60,20 @@ def test_whole_job_prices_each_launch_at_its_bound():
     t = 1e12 / bench.PEAK_FP64 + 8.0 * 3 * 10 ** 9 / bench.PEAK_HBM
     assert res["roofline_time_s"] == pytest.approx(t)
     assert res["frac"] == pytest.approx(t / 2.0)
+
+
+def test_profile_commands_run_the_committed_bench():
+    """Every `bench.py ...` command a profile summary gives parses with the
+    committed bench's command line (a summary must say how to make it again
+    with committed files)."""
+    import re
+    import shlex
+    pdir = os.path.join(ROOT, "profiles")
+    seen = 0
+    for f in sorted(os.listdir(pdir)):
+        for line in open(os.path.join(pdir, f)):
+            for m in re.finditer(r"bench\.py((?:\s+--?[\w-]+(?:\s+[^\s`-][^\s`]*)?)*)", line):
+                args = shlex.split(m.group(1))
+                bench.make_parser().parse_args(args)      # exits (SystemExit) on an unknown flag
+                seen += 1
+    assert seen >= 5
